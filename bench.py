@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""XS lookups/s on one MI355X: event-based macroscopic XS lookup, large problem
+(355 nuclides x 11,303 gridpoints, unionized grid, 17,000,000 lookups, FP64).
+
+A step is one xsb_hip_event() call over all 17M lookups of the reference's
+seeded sample stream (sampling, ordering and lookups, all on the device; the
+tables are resident in HBM before timing starts).  Every step's verification
+hash must be the published 952131 (openmp-threading/io.c:90) or the run fails.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+LOOKUPS = 17_000_000
+HASH_LARGE = 952131
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_lookup_large_unionized.txt")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profile(kernel_id):
+    """HBM bytes per lookup-kernel launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    vals = {}
+    for line in open(TRAFFIC_FILE):
+        m = re.match(r"^\s*([\w.]+)\s*[:=]\s*([-+0-9.eE]+)", line)
+        if m:
+            vals[m.group(1)] = float(m.group(2))
+    key = f"k{kernel_id}.hbm_bytes_per_launch"
+    return vals.get(key)
+
+
+def cpu_baseline(inp, sd):
+    """The reference's own openmp-threading event loop (oracle/_ref build) on
+    the same tables, or this project's C restatement where that build is
+    absent; all 17M lookups of the workload, one run."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    gomp = C.CDLL("libgomp.so.1")
+    gomp.omp_set_num_threads(threads)
+    oi = oracle.Inputs.from_buffer_copy(inp)
+    oi._hm = inp._hm
+    oi.HM = oi._hm
+    osd = oracle.SimulationData.from_buffer_copy(sd)
+    try:
+        ref = oracle.Reference()
+        kind = "reference"
+        t0 = time.perf_counter()
+        v = ref.run_event_based_simulation(oi, osd, 1)
+        dt = time.perf_counter() - t0
+    except (FileNotFoundError, OSError):
+        orc = oracle.Oracle()
+        kind = "port"
+        t0 = time.perf_counter()
+        v = orc.run_event(C.byref(oi), C.byref(osd), threads)
+        dt = time.perf_counter() - t0
+    if v % 999983 != HASH_LARGE:
+        raise SystemExit(f"CPU baseline produced hash {v % 999983}, expected {HASH_LARGE}")
+    return {"value": round(LOOKUPS / dt, 1), "unit": "lookups/s", "cores": threads, "kind": kind,
+            "sample": f"all {LOOKUPS:,} lookups of the workload, one run of "
+                      f"run_event_based_simulation (openmp-threading, schedule(dynamic,100)), "
+                      f"{dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--kernel", type=int, default=-1,
+                    help="lookup schedule (-k id); -1 picks the fastest exact FP64 schedule")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the lookup path is "
+              f"single-device here; see DESIGN.md for how it shards.", file=sys.stderr)
+        sys.exit(2)
+
+    import xsbench_amd as X
+
+    t0 = time.time()
+    inp = X.make_inputs("large", "unionized", "event")
+    sd = X.grid_init(inp)
+    log(f"host tables built in {time.time() - t0:.1f} s")
+    t0 = time.time()
+    dev = X.Device(inp, sd, device=0, build_fp32=True, build_bf16=True)
+    info = dev.info()
+    log(f"tables resident on {info['arch'].decode()} ({info['cu_count']} CUs) in {time.time() - t0:.1f} s")
+
+    def run(k, prec=X.XS_FP64):
+        r = dev.event(LOOKUPS, 0, kernel_id=k, precision=prec)
+        if r["hash"] != HASH_LARGE:
+            raise SystemExit(f"schedule {k} precision {prec}: hash {r['hash']} != {HASH_LARGE}")
+        return r
+
+    # pick the schedule: fastest exact FP64 schedule over two timed calls each
+    candidates = [args.kernel] if args.kernel >= 0 else [0, 1, 4, 6]
+    best, best_ms, sched_ms = None, 1e30, {}
+    for k in candidates:
+        run(k)
+        ms = min(run(k)["ms_total"] for _ in range(2))
+        sched_ms[k] = ms
+        log(f"schedule k={k}: {ms:.3f} ms per 17M lookups")
+        if ms < best_ms:
+            best, best_ms = k, ms
+    k = best
+
+    for _ in range(args.warmup):
+        run(k)
+    res = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res.append(run(k))
+    wall = time.perf_counter() - t0
+    value = LOOKUPS * args.steps / wall
+    ms_lookup = sum(r["ms_lookup"] for r in res) / len(res)
+    ms_total = sum(r["ms_total"] for r in res) / len(res)
+    log(f"k={k}: {value / 1e9:.3f} G lookups/s wall, GPU {ms_total:.3f} ms/step "
+        f"(lookup kernel {ms_lookup:.3f} ms)")
+
+    balg = dev.balg(LOOKUPS, precision=X.XS_FP64)
+    achieved = balg * LOOKUPS / (ms_lookup * 1e-3) / 1e9
+    traffic = traffic_from_profile(k)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "algorithmic_bytes_per_lookup": round(balg, 2),
+                "kernel": "k_lookup_ids" if k >= 2 else ("k_lookup_list" if k == 1 else "k_event_fused"),
+                "kernel_ms": round(ms_lookup, 4)}
+
+    variants = {f"fp64_k{kk}_lookups_per_s": round(LOOKUPS / (ms * 1e-3), 1) for kk, ms in sched_ms.items()}
+    if not args.no_variants:
+        for name, prec in (("fp32", X.XS_FP32), ("bf16", X.XS_BF16)):
+            run(6, prec)
+            rr = [run(6, prec) for _ in range(3)]
+            ms = min(r["ms_total"] for r in rr)
+            variants[f"{name}_k6_lookups_per_s"] = round(LOOKUPS / (ms * 1e-3), 1)
+            variants[f"{name}_k6_fp64_reevaluations"] = rr[0]["n_fallback"]
+            log(f"{name} k=6: {ms:.3f} ms, {rr[0]['n_fallback']} FP64 re-evaluations")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (reference openmp-threading loop on the host cores)...")
+        cpu = cpu_baseline(inp, sd)
+        log(f"CPU baseline: {cpu['value'] / 1e6:.2f} M lookups/s on {cpu['cores']} threads ({cpu['kind']})")
+
+    out = {
+        "metric": "XS lookups/sec, large 355-nuclide unionized grid (FP64 vs mixed); % HBM peak",
+        "value": round(value, 1),
+        "unit": "lookups/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (the reference's seeded generators: nuclide grid LCG seed 42, "
+                "lookup i sampled from FF(1070, 2i))",
+        "config": {"workload": "event-based macroscopic XS lookup, large problem: 355 nuclides x "
+                               "11303 gridpoints, unionized grid (4,012,565 energies, int32 index "
+                               "grid 5.7 GB), 17,000,000 lookups, FP64, hash 952131",
+                   "lookups": LOOKUPS, "grid": "unionized", "kernel_id": k,
+                   "gpu_ms_per_step": round(ms_total, 4)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "variants": variants,
+    }
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

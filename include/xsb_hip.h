@@ -1,0 +1,194 @@
+/*
+ * xsb_hip.h -- C ABI of libxsb_hip.so, the MI355X (gfx950) implementation of
+ * XSBench's macroscopic cross-section lookup.
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, opaque handles,
+ * integer return codes (never exit()).  No HIP or torch types appear here;
+ * a stream argument is a hipStream_t passed as void* (NULL = the handle's
+ * own stream).  One handle owns one device copy of the tables and one
+ * stream; handles on different devices are independent and may be used
+ * from different host threads.  Calls on one handle must not overlap.
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef XSB_HIP_H
+#define XSB_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ---- */
+#define XSB_OK 0
+#define XSB_ERR_INVALID 1     /* bad argument / inconsistent sizes */
+#define XSB_ERR_NO_DEVICE 2   /* no usable HIP device */
+#define XSB_ERR_OOM 3         /* device allocation failed */
+#define XSB_ERR_HIP 4         /* a HIP runtime call failed */
+#define XSB_ERR_UNSUPPORTED 5 /* option not available for this table set */
+
+/* ---- grid types: same values as openmp-threading/XSbench_header.h:37-39 ---- */
+#define XSB_UNIONIZED 0
+#define XSB_NUCLIDE 1
+#define XSB_HASH 2
+
+/* ---- XS table precision ---- */
+#define XSB_XS_FP64 0 /* reference tables, bit-exact */
+#define XSB_XS_FP32 1 /* FP64 energies + FP32 XS, FP32 Kahan sums, FP64 fallback */
+#define XSB_XS_BF16 2 /* FP64 energies + BF16 XS scaled per nuclide, same fallback */
+
+/* ---- upload flags ---- */
+#define XSB_BUILD_FP32 0x1u /* also build the FP32 XS table on the device */
+#define XSB_BUILD_BF16 0x2u /* also build the BF16 XS table on the device */
+
+/* ---- lookup schedules: the reference's -k kernel ids
+ *      (openmp-threading/Main.c:80-93, cuda/Main.cu:62-83) ---- */
+#define XSB_K_BASELINE 0     /* sample + lookup fused, one lookup per lane, lookup order */
+#define XSB_K_SPLIT 1        /* sampling kernel, then lookup kernel (cuda opt 1) */
+#define XSB_K_MAT_LAUNCH 2   /* samples partitioned by material, one launch per material (opt 2) */
+#define XSB_K_FUEL_LAUNCH 3  /* fuel / non-fuel partition, two launches (opt 3) */
+#define XSB_K_MAT_SORT 4     /* material sort, per-material segments (opt 4) */
+#define XSB_K_FUEL_SORT 5    /* fuel-first partition, two segments (opt 5) */
+#define XSB_K_MAT_E_SORT 6   /* material then energy sort, per-material segments (opt 6) */
+#define XSB_K_AUTO -1        /* fastest exact schedule for the table set */
+
+/* Host view of the reference's SimulationData (openmp-threading/
+ * XSbench_header.h:77-102) plus the Inputs fields the lookup needs. */
+typedef struct {
+	int64_t n_isotopes;
+	int64_t n_gridpoints;
+	int32_t grid_type;
+	int32_t hash_bins;
+	const int32_t *num_nucs; /* 12 */
+	const int32_t *mats;     /* 12 x max_num_nucs */
+	const double *concs;     /* 12 x max_num_nucs */
+	int32_t max_num_nucs;
+	int32_t reserved0;
+	const double *unionized_energy_array; /* length_unionized_energy_array */
+	int64_t length_unionized_energy_array;
+	const int32_t *index_grid; /* length_index_grid */
+	int64_t length_index_grid;
+	const double *nuclide_grid; /* n_isotopes x n_gridpoints points of 6 doubles */
+	int64_t length_nuclide_grid;
+} xsb_tables;
+
+/* Event-based run (openmp-threading/Simulation.c:15-114) */
+typedef struct {
+	int64_t lookups;      /* number of lookups */
+	int64_t first_lookup; /* id of the first lookup in the sample stream */
+	uint64_t seed;        /* STARTING_SEED (1070) in the reference */
+	int32_t kernel_id;    /* XSB_K_* */
+	int32_t xs_precision; /* XSB_XS_* */
+	/* optional per-lookup results, host pointers of length `lookups`
+	 * (macro: lookups x 5); NULL skips the output */
+	double *out_energy;
+	int32_t *out_mat;
+	int64_t *out_index; /* unionized-grid index, or hash bin */
+	double *out_macro;
+	int32_t *out_argmax;
+} xsb_event_args;
+
+/* History-based run (openmp-threading/Simulation.c:116-238) */
+typedef struct {
+	int64_t particles;
+	int32_t lookups_per_particle;
+	int32_t kernel_id;    /* 0: one particle per lane; 6: rounds, material-sorted */
+	int32_t xs_precision; /* XSB_XS_* */
+	int32_t reserved0;
+	uint64_t seed;                 /* STARTING_SEED */
+	uint64_t *out_particle_sum;    /* optional host array [particles] */
+	uint64_t *out_final_seed;      /* optional host array [particles] */
+} xsb_history_args;
+
+/* A set of independent lookup batches, each with its own sample stream and
+ * material CDF (no reference counterpart; the producer of many small event
+ * batches).  Batch b holds sizes[b] lookups; lookup i of batch b samples
+ * s = FF(seeds[b], 2i), E = LCG(s), roll = LCG(s), mat = first m with
+ * roll < thresholds[12 b + m]. */
+typedef struct {
+	int32_t n_batches;
+	int32_t xs_precision;
+	const int64_t *sizes;      /* host [n_batches] */
+	const uint64_t *seeds;     /* host [n_batches] */
+	const double *thresholds;  /* host [n_batches x 12], thresholds[12b+11] >= 1 */
+	int32_t mode;              /* 0 persistent kernel + graph, 1 one launch per batch */
+	int32_t tile;              /* lookups per work item (0 = autotuned) */
+	uint64_t *out_vsum;        /* optional host [n_batches] */
+	double *out_done_ms;       /* optional host [n_batches]: completion time of each batch */
+} xsb_batched_args;
+
+typedef struct {
+	uint64_t vsum;        /* un-modded verification sum (main applies % 999983) */
+	double ms_total;      /* GPU time of every kernel of the call (hipEvents) */
+	double ms_lookup;     /* GPU time of the lookup kernel(s) */
+	double ms_sample;     /* sampling / sort kernels */
+	int64_t n_fallback;   /* lookups re-evaluated in FP64 (mixed precision) */
+	int64_t material_counts[12];
+	int32_t kernel_id;    /* schedule that ran */
+	int32_t reserved0;
+} xsb_result;
+
+typedef struct {
+	int32_t device;
+	int32_t grid_type;
+	int64_t n_isotopes;
+	int64_t n_gridpoints;
+	int64_t n_union;
+	int64_t bytes_fp64;       /* FP64 AoS grid */
+	int64_t bytes_fp32;       /* FP32 XS table (0 if not built) */
+	int64_t bytes_bf16;       /* BF16 XS table (0 if not built) */
+	int64_t bytes_index;      /* index grid */
+	int64_t bytes_union;      /* unionized energies */
+	int32_t cu_count;
+	int32_t xcd_count;
+	char arch[32];
+} xsb_info;
+
+/* cuda/GridInit.cu:4 move_simulation_data_to_device: copy the tables to
+ * `device` (hipMalloc + hipMemcpy, FP64 AoS + index grid + materials), and
+ * build the compact XS tables named by `flags` on the device. */
+int xsb_hip_upload(int device, const xsb_tables *t, uint32_t flags, void **handle);
+
+/* cuda/GridInit.cu:81 release_device_memory: free everything the handle owns. */
+int xsb_hip_release(void *handle);
+
+/* openmp-threading/Simulation.c:15 run_event_based_simulation and
+ * Simulation.c:698 / cuda/Simulation.cu:388-1099 run_event_based_simulation_
+ * optimization_k, selected by args->kernel_id. */
+int xsb_hip_event(void *handle, const xsb_event_args *args, xsb_result *res);
+
+/* openmp-threading/Simulation.c:116 run_history_based_simulation */
+int xsb_hip_history(void *handle, const xsb_history_args *args, xsb_result *res);
+
+/* openmp-threading/Simulation.c:326 calculate_macro_xs over n explicit
+ * samples held in device memory: d_energy[n], d_mat[n] -> d_macro[n x 5]
+ * (nullable), d_argmax[n] (nullable), *d_vsum += sum(argmax + 1) (nullable).
+ * Enqueued on `stream` (NULL = the handle's stream); does not synchronize. */
+int xsb_hip_lookup(void *handle, int64_t n, const double *d_energy, const int32_t *d_mat,
+                   int32_t xs_precision, double *d_macro, int32_t *d_argmax,
+                   unsigned long long *d_vsum, void *stream);
+
+/* Many independent batches (see xsb_batched_args); no reference counterpart. */
+int xsb_hip_batched(void *handle, const xsb_batched_args *args, xsb_result *res);
+
+/* Mean algorithmic bytes per lookup of an event run (the stored size of the
+ * distinct words the reference algorithm reads), counted on the device for
+ * the table format of `xs_precision`. */
+int xsb_hip_balg(void *handle, const xsb_event_args *args, double *bytes_per_lookup);
+
+/* Device copy bandwidth probe: copies `bytes` once per rep with a plain
+ * vector copy kernel and returns the mean kernel time (for scaling HBM
+ * counters). */
+int xsb_hip_copy_probe(void *handle, int64_t bytes, int32_t reps, double *ms_per_copy);
+
+int xsb_hip_info(void *handle, xsb_info *info);
+const char *xsb_hip_strerror(int code);
+/* message of the last failure on the calling thread ("" if none) */
+const char *xsb_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,299 @@
+"""GPU parity: the HIP path through the C ABI against the reference's golden
+vectors, the published hashes and the oracle, bit for bit in FP64.
+
+Every test is marked gpu; run on an MI355X with
+    python -u -m pytest tests -m gpu -x -v --timeout 120 --timeout-method thread
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import xsbench_amd as X
+from conftest import DATA_CFGS, golden_npz
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = [X.K_BASELINE, X.K_SPLIT, X.K_MAT_LAUNCH, X.K_FUEL_LAUNCH, X.K_MAT_SORT, X.K_FUEL_SORT,
+           X.K_MAT_E_SORT]
+
+_CACHE = {}
+
+
+def data(name):
+    """(inputs, host SD, device handle) per golden data configuration, built
+    once per session; every device copy also holds the FP32 and BF16 tables."""
+    if name not in _CACHE:
+        size, grid, gp = DATA_CFGS[name]
+        inp = X.make_inputs(size, grid, "event", gridpoints=gp)
+        sd = X.grid_init(inp)
+        print(f"[gpu-test] upload {name}", flush=True)
+        dev = X.Device(inp, sd, build_fp32=True, build_bf16=True)
+        _CACHE[name] = (inp, sd, dev)
+    return _CACHE[name]
+
+
+def as_oracle(inp, sd):
+    """the oracle's view of the product's tables (same struct layout)"""
+    oi = O.Inputs.from_buffer_copy(inp)
+    oi._hm = inp._hm
+    oi.HM = oi._hm
+    return oi, O.SimulationData.from_buffer_copy(sd)
+
+
+def same_bits(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint64), np.ascontiguousarray(b).view(np.uint64))
+
+
+# ---------------------------------------------------------------------------
+# per-lookup parity
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", list(DATA_CFGS))
+@pytest.mark.parametrize("k", KERNELS)
+def test_first_lookups_match_reference_vectors(name, k):
+    inp, sd, dev = data(name)
+    g = golden_npz(f"lookups_{name}.npz")
+    n = 1024
+    assert np.array_equal(g["ids"][:n], np.arange(n))
+    res, out = dev.event(n, 0, kernel_id=k, detail=True)
+    assert same_bits(out["energy"], g["energy"][:n])
+    assert np.array_equal(out["mat"], g["mat"][:n])
+    if inp.grid_type == X.UNIONIZED:
+        assert np.array_equal(out["index"], g["ueg_idx"][:n])
+    assert same_bits(out["macro"], g["macro"][:n])
+    assert np.array_equal(out["argmax"], g["argmax"][:n])
+    assert res["vsum"] == int(g["argmax"][:n].sum()) + n
+
+
+@pytest.mark.parametrize("name", list(DATA_CFGS))
+def test_spread_lookups_match_reference_vectors(name):
+    inp, sd, dev = data(name)
+    g = golden_npz(f"lookups_{name}.npz")
+    ids = g["ids"][1024:]
+    for j in range(0, len(ids), 16):
+        for q, i in enumerate(ids[j:j + 16]):
+            _, out = dev.event(1, int(i), kernel_id=X.K_BASELINE, detail=True)
+            assert same_bits(out["macro"][0], g["macro"][1024 + j + q])
+            assert out["argmax"][0] == g["argmax"][1024 + j + q]
+
+
+@pytest.mark.parametrize("name", ["small_unionized", "large_unionized", "large_hash", "small_nuclide"])
+@pytest.mark.parametrize("k", [X.K_BASELINE, X.K_SPLIT, X.K_MAT_E_SORT])
+def test_window_matches_oracle(orc, name, k):
+    inp, sd, dev = data(name)
+    oi, osd = as_oracle(inp, sd)
+    first, n = 5_000_003, 1 << 15
+    _, out = dev.event(n, first, kernel_id=k, detail=True)
+    E, m, idx, macro, am = orc.event_detail(oi, osd, np.arange(first, first + n))
+    assert same_bits(out["energy"], E) and np.array_equal(out["mat"], m)
+    assert same_bits(out["macro"], macro) and np.array_equal(out["argmax"], am)
+
+
+# ---------------------------------------------------------------------------
+# full-size hashes
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("k", KERNELS)
+def test_large_unionized_event_hash_every_schedule(k):
+    inp, sd, dev = data("large_unionized")
+    r = dev.event(17_000_000, 0, kernel_id=k)
+    print(f"[gpu-test] large unionized k={k}: {r['ms_total']:.2f} ms", flush=True)
+    assert r["hash"] == X.PUBLISHED[("event", "large")]
+    if k in (X.K_MAT_SORT, X.K_MAT_E_SORT):
+        assert sum(r["material_counts"]) == 17_000_000
+
+
+@pytest.mark.parametrize("name,k", [("large_hash", X.K_BASELINE), ("large_hash", X.K_MAT_E_SORT),
+                                    ("small_unionized", X.K_BASELINE), ("small_hash", X.K_MAT_E_SORT),
+                                    ("small_nuclide", X.K_BASELINE), ("small_nuclide", X.K_MAT_E_SORT)])
+def test_event_hash_grid_invariance(name, k):
+    inp, sd, dev = data(name)
+    r = dev.event(17_000_000, 0, kernel_id=k)
+    size = DATA_CFGS[name][0]
+    assert r["hash"] == X.PUBLISHED[("event", size)]
+
+
+def test_small_one_million_lookups_golden(golden_hashes):
+    inp, sd, dev = data("small_unionized")
+    for k in (X.K_BASELINE, X.K_MAT_E_SORT):
+        assert dev.event(1_000_000, 0, kernel_id=k)["vsum"] == golden_hashes["event_small_unionized_1M"]["vsum"]
+
+
+# ---------------------------------------------------------------------------
+# history mode
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["small_unionized", "small_hash", "large_hash", "xl45k_hash"])
+def test_history_particles_match_reference(name):
+    inp, sd, dev = data(name)
+    g = golden_npz(f"history_{name}.npz")
+    n = len(g["sums"])
+    r, out = dev.history(n, int(g["lookups"]), detail=True)
+    assert np.array_equal(out["sums"], g["sums"])
+    assert np.array_equal(out["final_seed"], g["final_seed"])
+
+
+@pytest.mark.parametrize("name,size", [("small_unionized", "small"), ("large_unionized", "large")])
+def test_history_published_hashes(name, size):
+    inp, sd, dev = data(name)
+    r = dev.history(500_000, 34)
+    print(f"[gpu-test] history {name}: {r['ms_total']:.2f} ms", flush=True)
+    assert r["hash"] == X.PUBLISHED[("history", size)]
+
+
+def test_history_xl45k_golden(golden_hashes):
+    inp, sd, dev = data("xl45k_hash")
+    for key in ("history_xl45k_hash_20k", "history_xl45k_hash_1M"):
+        h = golden_hashes[key]
+        assert dev.history(h["particles"], h["lookups"])["vsum"] == h["vsum"]
+
+
+# ---------------------------------------------------------------------------
+# mixed precision: identical decisions, bounded error
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", [X.XS_FP32, X.XS_BF16])
+@pytest.mark.parametrize("name", ["large_unionized", "large_hash"])
+def test_mixed_precision_event_hash_identical(name, prec):
+    inp, sd, dev = data(name)
+    for k in (X.K_SPLIT, X.K_MAT_E_SORT):
+        r = dev.event(17_000_000, 0, kernel_id=k, precision=prec)
+        print(f"[gpu-test] {name} prec={prec} k={k}: {r['ms_total']:.2f} ms, "
+              f"{r['n_fallback']} FP64 re-evaluations", flush=True)
+        assert r["hash"] == X.PUBLISHED[("event", "large")]
+
+
+@pytest.mark.parametrize("prec,tol", [(X.XS_FP32, 1e-6), (X.XS_BF16, 2e-2)])
+def test_mixed_precision_error_per_channel(prec, tol):
+    inp, sd, dev = data("large_hash")
+    n = 1 << 18
+    _, ref = dev.event(n, 0, kernel_id=X.K_SPLIT, precision=X.XS_FP64, detail=True)
+    _, mix = dev.event(n, 0, kernel_id=X.K_SPLIT, precision=prec, detail=True)
+    assert np.array_equal(ref["argmax"], mix["argmax"])
+    rel = np.abs(mix["macro"] - ref["macro"]) / np.abs(ref["macro"])
+    print(f"[gpu-test] prec={prec} max rel err per channel {rel.max(axis=0)}, mean {rel.mean(axis=0)}",
+          flush=True)
+    assert np.all(rel.max(axis=0) < tol)
+
+
+@pytest.mark.parametrize("prec", [X.XS_FP32, X.XS_BF16])
+def test_mixed_precision_history_identical(prec):
+    inp, sd, dev = data("large_hash")
+    r64 = dev.history(20000, 34)
+    r = dev.history(20000, 34, precision=prec)
+    assert r["vsum"] == r64["vsum"]
+
+
+# ---------------------------------------------------------------------------
+# explicit-sample API on device memory, batches, counters, errors
+# ---------------------------------------------------------------------------
+def _hip_runtime():
+    """the HIP runtime libxsb_hip.so itself links (already loaded)"""
+    X.hip_lib()
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipDeviceSynchronize.argtypes = []
+    return hip
+
+
+def test_lookup_on_device_buffers(orc):
+    inp, sd, dev = data("small_hash")
+    oi, osd = as_oracle(inp, sd)
+    hip = _hip_runtime()
+    rng = np.random.default_rng(17)
+    n = 50000
+    E = rng.random(n)
+    M = rng.integers(0, 12, n).astype(np.int32)
+    ptrs = [C.c_void_p() for _ in range(5)]
+    sizes = [n * 8, n * 4, n * 40, n * 4, 8]
+    for p_, b in zip(ptrs, sizes):
+        assert hip.hipMalloc(C.byref(p_), b) == 0
+    try:
+        H2D, D2H = 1, 2
+        assert hip.hipMemcpy(ptrs[0], E.ctypes.data, n * 8, H2D) == 0
+        assert hip.hipMemcpy(ptrs[1], M.ctypes.data, n * 4, H2D) == 0
+        assert hip.hipMemset(ptrs[4], 0, 8) == 0
+        dev.lookup(n, ptrs[0], ptrs[1], X.XS_FP64, ptrs[2], ptrs[3], ptrs[4], None)
+        assert hip.hipDeviceSynchronize() == 0
+        macro = np.empty((n, 5))
+        am = np.empty(n, np.int32)
+        vs = np.zeros(1, np.uint64)
+        assert hip.hipMemcpy(macro.ctypes.data, ptrs[2], n * 40, D2H) == 0
+        assert hip.hipMemcpy(am.ctypes.data, ptrs[3], n * 4, D2H) == 0
+        assert hip.hipMemcpy(vs.ctypes.data, ptrs[4], 8, D2H) == 0
+    finally:
+        for p_ in ptrs:
+            hip.hipFree(p_)
+    want_macro, want_am = orc.lookup_detail(oi, osd, E, M)
+    assert same_bits(macro, want_macro)
+    assert np.array_equal(am, want_am)
+    assert int(vs[0]) == int(want_am.sum()) + n
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("prec", [X.XS_FP64, X.XS_FP32])
+def test_batched_per_batch_hashes(orc, mode, prec):
+    inp, sd, dev = data("small_unionized")
+    oi, osd = as_oracle(inp, sd)
+    sizes, seeds, thr = orc.batch_plan(2024, 24, 1000, 60000)
+    r, vs, done = dev.batched(sizes, seeds, thr, precision=prec, mode=mode)
+    for b in range(len(sizes)):
+        assert int(vs[b]) == orc.run_batch(oi, osd, int(seeds[b]), thr[b], int(sizes[b]))
+    assert r["vsum"] == int(vs.sum())
+    assert np.all(done > 0)
+
+
+@pytest.mark.parametrize("name", ["small_unionized", "small_hash", "small_nuclide"])
+def test_balg_counter_matches_oracle(orc, name):
+    inp, sd, dev = data(name)
+    oi, osd = as_oracle(inp, sd)
+    n = 20000
+    assert dev.balg(n) == pytest.approx(orc.balg_event(oi, osd, 0, n), rel=1e-12)
+
+
+def test_error_paths_on_device():
+    inp = X.make_inputs("small", "hash", "event", gridpoints=50)
+    sd = X.grid_init(inp)
+    with X.Device(inp, sd) as dev:  # no compact tables
+        with pytest.raises(X.XsbError) as e:
+            dev.event(100, precision=X.XS_FP32)
+        assert e.value.code == 5
+        with pytest.raises(X.XsbError) as e:
+            dev.event(100, kernel_id=11)
+        assert e.value.code == 1
+        with pytest.raises(X.XsbError) as e:
+            dev.event(-5)
+        assert e.value.code == 1
+        assert dev.event(0)["vsum"] == 0
+        assert dev.event(0, kernel_id=X.K_MAT_E_SORT)["vsum"] == 0
+
+
+# ---------------------------------------------------------------------------
+# the reference's host surface end to end
+# ---------------------------------------------------------------------------
+def test_host_drivers_return_published_hash():
+    lib = X.host_lib()
+    inp = X.make_inputs("small", "unionized", "event")
+    sd = X.grid_init(inp)
+    # unstaged SD: the driver uploads a temporary copy
+    assert lib.run_event_based_simulation(inp, sd, 1) % 999983 == X.PUBLISHED[("event", "small")]
+    lib.move_simulation_data_to_device(inp, 1, sd)
+    inp.kernel_id = 6
+    assert lib.run_event_based_simulation(inp, sd, 1) % 999983 == X.PUBLISHED[("event", "small")]
+    assert lib.run_event_based_simulation_optimization_1(inp, sd, 1) % 999983 == X.PUBLISHED[("event", "small")]
+    hin = X.make_inputs("small", "unionized", "history")
+    assert lib.run_history_based_simulation(hin, sd, 1) % 999983 == X.PUBLISHED[("history", "small")]
+    lib.release_device_memory(sd)
+
+
+@pytest.mark.parametrize("args", [["-m", "event", "-s", "small"],
+                                  ["-m", "event", "-s", "small", "-G", "hash", "-k", "6", "-P", "fp32"],
+                                  ["-m", "history", "-s", "small", "-G", "nuclide"]])
+def test_program_flow_valid_checksum(capfd, args):
+    """openmp-threading/Main.c flow in-process (no exec from a GPU process)"""
+    rc = X.run_main(args)
+    out = capfd.readouterr().out
+    assert rc == 0, out[-3000:]
+    assert "(Valid)" in out

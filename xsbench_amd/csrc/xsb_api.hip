@@ -1,0 +1,730 @@
+// xsb_api.hip -- the extern "C" entry points of libxsb_hip.so (xsb_hip.h).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "xsb_internal.h"
+#include "xsb_mixed.h"
+
+namespace xsb {
+
+static thread_local char g_msg[1024];
+
+int fail(int code, const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_msg, sizeof(g_msg), fmt, ap);
+	va_end(ap);
+	return code;
+}
+
+int scratch_reserve(Device *d, int slot, size_t bytes, void **out)
+{
+	Scratch &s = d->scratch[slot];
+	if (s.bytes < bytes) {
+		if (s.ptr) {
+			XSB_HIP(hipStreamSynchronize(d->stream));
+			XSB_HIP(hipFree(s.ptr));
+			s.ptr = nullptr;
+			s.bytes = 0;
+		}
+		const size_t want = (bytes + 0xFFFFF) & ~(size_t)0xFFFFF;
+		if (hipMalloc(&s.ptr, want) != hipSuccess) {
+			(void)hipGetLastError();
+			return fail(XSB_ERR_OOM, "device scratch allocation of %zu bytes failed", want);
+		}
+		s.bytes = want;
+	}
+	*out = s.ptr;
+	return XSB_OK;
+}
+
+// Volume fractions of the 12 materials (openmp-threading/Simulation.c:433-445)
+// and pick_mat's thresholds: dist[i] + dist[i-1] + ... + dist[1], summed in
+// that order (Simulation.c:450-457).
+static void pick_thresholds(double thr[kMaterials])
+{
+	static const double dist[kMaterials] = {0.140, 0.052, 0.275, 0.134, 0.154, 0.064,
+	                                        0.066, 0.055, 0.008, 0.015, 0.025, 0.013};
+	for (int i = 0; i < kMaterials; i++) {
+		double running = 0;
+		for (int j = i; j > 0; j--)
+			running += dist[j];
+		thr[i] = running;
+	}
+}
+
+template <typename T>
+static int upload_array(T **dst, const T *src, int64_t count, int64_t *bytes_acc)
+{
+	*dst = nullptr;
+	if (count <= 0)
+		return XSB_OK;
+	const size_t bytes = (size_t)count * sizeof(T);
+	if (hipMalloc((void **)dst, bytes) != hipSuccess) {
+		(void)hipGetLastError();
+		*dst = nullptr;
+		return fail(XSB_ERR_OOM, "hipMalloc of %zu bytes failed", bytes);
+	}
+	XSB_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+	if (bytes_acc)
+		*bytes_acc += (int64_t)bytes;
+	return XSB_OK;
+}
+
+static void release(Device *d)
+{
+	if (!d)
+		return;
+	// best effort: a failure here leaves nothing the caller could act on
+	(void)hipSetDevice(d->device);
+	if (d->stream)
+		(void)hipStreamSynchronize(d->stream);
+	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_grid,  d->d_egrid,    d->d_xs32, d->d_xs16,
+	                d->d_xs16_scale, d->d_num_nucs, d->d_mats, d->d_concs, d->d_concs32, d->d_acc};
+	for (void *p : ptrs)
+		if (p)
+			(void)hipFree(p);
+	for (auto &s : d->scratch)
+		if (s.ptr)
+			(void)hipFree(s.ptr);
+	for (auto &e : d->ev)
+		if (e)
+			(void)hipEventDestroy(e);
+	if (d->stream)
+		(void)hipStreamDestroy(d->stream);
+	(void)hipGetLastError();
+	delete d;
+}
+
+// Device-side range check of the index grid: every entry must lie in
+// [0, n_gp - 1] (grid construction never writes more than n_gp - 2).
+__global__ void k_index_range(const int32_t *__restrict__ idx, int64_t n, int32_t hi,
+                              unsigned long long *bad)
+{
+	uint32_t b = 0;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const int32_t v = idx[i];
+		b += (v < 0 || v > hi) ? 1u : 0u;
+	}
+	b = wave_sum_u32(b);
+	if ((threadIdx.x & 63) == 0 && b)
+		atomicAdd(bad, (unsigned long long)b);
+}
+
+// Unionized energies must be non-decreasing for the bisection to be exact.
+__global__ void k_sorted_check(const double *__restrict__ a, int64_t n, unsigned long long *bad)
+{
+	uint32_t b = 0;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n;
+	     i += (int64_t)gridDim.x * blockDim.x)
+		b += (a[i] > a[i + 1]) ? 1u : 0u;
+	b = wave_sum_u32(b);
+	if ((threadIdx.x & 63) == 0 && b)
+		atomicAdd(bad, (unsigned long long)b);
+}
+
+static int validate_tables(const xsb_tables *t)
+{
+	if (!t)
+		return fail(XSB_ERR_INVALID, "tables is NULL");
+	if (t->n_isotopes < 1 || t->n_gridpoints < 2)
+		return fail(XSB_ERR_INVALID, "n_isotopes=%lld n_gridpoints=%lld out of range",
+		            (long long)t->n_isotopes, (long long)t->n_gridpoints);
+	if ((int64_t)t->n_isotopes * t->n_gridpoints > INT32_MAX)
+		return fail(XSB_ERR_INVALID, "n_isotopes * n_gridpoints exceeds 2^31-1");
+	if (t->grid_type < 0 || t->grid_type > 2)
+		return fail(XSB_ERR_INVALID, "grid_type %d is not 0, 1 or 2", t->grid_type);
+	if (!t->num_nucs || !t->mats || !t->concs || !t->nuclide_grid)
+		return fail(XSB_ERR_INVALID, "num_nucs, mats, concs and nuclide_grid are required");
+	if (t->length_nuclide_grid != t->n_isotopes * t->n_gridpoints)
+		return fail(XSB_ERR_INVALID, "length_nuclide_grid %lld != n_isotopes * n_gridpoints",
+		            (long long)t->length_nuclide_grid);
+	if (t->max_num_nucs < 1)
+		return fail(XSB_ERR_INVALID, "max_num_nucs must be positive");
+	for (int m = 0; m < kMaterials; m++) {
+		const int nn = t->num_nucs[m];
+		if (nn < 0 || nn > t->max_num_nucs)
+			return fail(XSB_ERR_INVALID, "num_nucs[%d]=%d outside [0, max_num_nucs]", m, nn);
+		for (int j = 0; j < nn; j++) {
+			const int nuc = t->mats[m * t->max_num_nucs + j];
+			if (nuc < 0 || nuc >= t->n_isotopes)
+				return fail(XSB_ERR_INVALID, "mats[%d][%d]=%d is not a nuclide id", m, j, nuc);
+		}
+	}
+	if (t->grid_type == XSB_UNIONIZED) {
+		if (!t->unionized_energy_array || !t->index_grid)
+			return fail(XSB_ERR_INVALID, "unionized grid needs unionized_energy_array and index_grid");
+		if (t->length_unionized_energy_array != t->n_isotopes * t->n_gridpoints)
+			return fail(XSB_ERR_INVALID, "unionized grid length %lld != n_isotopes * n_gridpoints",
+			            (long long)t->length_unionized_energy_array);
+		if (t->length_index_grid != t->length_unionized_energy_array * t->n_isotopes)
+			return fail(XSB_ERR_INVALID, "index grid length %lld != n_union * n_isotopes",
+			            (long long)t->length_index_grid);
+	} else if (t->grid_type == XSB_HASH) {
+		if (!t->index_grid || t->hash_bins < 1)
+			return fail(XSB_ERR_INVALID, "hash grid needs index_grid and hash_bins >= 1");
+		if (t->length_index_grid != (int64_t)t->hash_bins * t->n_isotopes)
+			return fail(XSB_ERR_INVALID, "hash index grid length %lld != hash_bins * n_isotopes",
+			            (long long)t->length_index_grid);
+	}
+	return XSB_OK;
+}
+
+// float rounded up (towards +inf) for bound constants
+static float f32_up(double x)
+{
+	float f = (float)x;
+	if ((double)f < x)
+		f = std::nextafter(f, INFINITY);
+	return f;
+}
+
+__global__ void k_build_fp32(const double *__restrict__ g, int64_t n, double *__restrict__ rec)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const double *p = g + i * kPointDoubles;
+		float4 x = make_float4((float)p[1], (float)p[2], (float)p[3], (float)p[4]);
+		double *r = rec + i * kRec32Doubles;
+		r[0] = p[0];
+		*reinterpret_cast<float4 *>(r + 1) = x;
+		*reinterpret_cast<float2 *>(r + 3) = make_float2((float)p[5], 0.f);
+	}
+}
+
+// round-to-nearest-even float -> bf16 (finite inputs)
+__device__ __forceinline__ uint32_t f32_to_bf16(float f)
+{
+	uint32_t u = __float_as_uint(f);
+	u += 0x7FFFu + ((u >> 16) & 1u);
+	return u >> 16;
+}
+
+__global__ void k_build_bf16(const double *__restrict__ g, int64_t n, int32_t n_gp,
+                             const float *__restrict__ inv_scale, double *__restrict__ rec)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const double *p = g + i * kPointDoubles;
+		const double is = (double)inv_scale[i / n_gp];  // power of two: exact
+		uint32_t h[5];
+		for (int c = 0; c < 5; c++)
+			h[c] = f32_to_bf16((float)(p[1 + c] * is));
+		double *r = rec + i * kRec16Doubles;
+		r[0] = p[0];
+		*reinterpret_cast<uint2 *>(r + 1) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+		*reinterpret_cast<uint2 *>(r + 2) = make_uint2(h[4], 0u);
+	}
+}
+
+static int build_compact(Device *d, const xsb_tables *t, uint32_t flags)
+{
+	const int64_t n_iso = t->n_isotopes, n_gp = t->n_gridpoints, n = n_iso * n_gp;
+	const int max_n = t->max_num_nucs;
+	const int plane = kMaterials * max_n;
+	// per nuclide: max |xs| over points and channels
+	std::vector<double> mx(n_iso, 0.0);
+	for (int64_t nuc = 0; nuc < n_iso; nuc++) {
+		double m = 0.0;
+		const double *p = t->nuclide_grid + nuc * n_gp * kPointDoubles;
+		for (int64_t k = 0; k < n_gp; k++)
+			for (int c = 1; c < kPointDoubles; c++)
+				m = std::fmax(m, std::fabs(p[k * kPointDoubles + c]));
+		if (!std::isfinite(m))
+			return fail(XSB_ERR_INVALID, "nuclide %lld holds non-finite XS", (long long)nuc);
+		mx[nuc] = m;
+	}
+	// power-of-two scale per nuclide with max|xs| <= scale (BF16 table)
+	std::vector<float> scale(n_iso), inv_scale(n_iso);
+	for (int64_t nuc = 0; nuc < n_iso; nuc++) {
+		int e = 0;
+		if (mx[nuc] > 0) {
+			std::frexp(mx[nuc], &e);  // mx = f * 2^e, f in [0.5, 1)
+		}
+		scale[nuc] = std::ldexp(1.0f, e);
+		inv_scale[nuc] = std::ldexp(1.0f, -e);
+	}
+	// constant planes: FP32 c_eff, FP32 c_bound, BF16 c_eff, BF16 c_bound
+	std::vector<float> planes(4 * (size_t)plane, 0.f);
+	for (int m = 0; m < kMaterials; m++) {
+		for (int j = 0; j < t->num_nucs[m]; j++) {
+			const int nuc = t->mats[m * max_n + j];
+			const double conc = t->concs[m * max_n + j];
+			const size_t o = (size_t)m * max_n + j;
+			planes[o] = (float)conc;
+			planes[plane + o] = f32_up(std::fabs(conc) * mx[nuc] * (1.0 + 0x1p-20));
+			planes[2 * plane + o] = (float)(conc * (double)scale[nuc]);
+			planes[3 * plane + o] = f32_up(std::fabs(conc) * (double)scale[nuc] * (1.0 + 0x1p-7));
+		}
+	}
+	int rc = upload_array(&d->d_concs32, planes.data(), (int64_t)planes.size(), nullptr);
+	if (rc)
+		return rc;
+	const unsigned nb = 2048;
+	if (flags & XSB_BUILD_FP32) {
+		const size_t bytes = (size_t)n * kRec32Doubles * sizeof(double);
+		if (hipMalloc((void **)&d->d_xs32, bytes) != hipSuccess) {
+			(void)hipGetLastError();
+			d->d_xs32 = nullptr;
+			return fail(XSB_ERR_OOM, "FP32 table allocation (%zu bytes) failed", bytes);
+		}
+		hipLaunchKernelGGL(k_build_fp32, dim3(nb), dim3(256), 0, d->stream, d->d_grid, n,
+		                   (double *)d->d_xs32);
+		XSB_HIP(hipGetLastError());
+		d->bytes_fp32 = (int64_t)bytes;
+	}
+	if (flags & XSB_BUILD_BF16) {
+		const size_t bytes = (size_t)n * kRec16Doubles * sizeof(double);
+		if (hipMalloc((void **)&d->d_xs16, bytes) != hipSuccess) {
+			(void)hipGetLastError();
+			d->d_xs16 = nullptr;
+			return fail(XSB_ERR_OOM, "BF16 table allocation (%zu bytes) failed", bytes);
+		}
+		rc = upload_array(&d->d_xs16_scale, inv_scale.data(), n_iso, nullptr);
+		if (rc)
+			return rc;
+		hipLaunchKernelGGL(k_build_bf16, dim3(nb), dim3(256), 0, d->stream, d->d_grid, n,
+		                   (int32_t)n_gp, d->d_xs16_scale, (double *)d->d_xs16);
+		XSB_HIP(hipGetLastError());
+		d->bytes_bf16 = (int64_t)bytes;
+	}
+	XSB_HIP(hipStreamSynchronize(d->stream));
+	return XSB_OK;
+}
+
+static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
+{
+	int rc = validate_tables(t);
+	if (rc)
+		return rc;
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+		(void)hipGetLastError();
+		return fail(XSB_ERR_NO_DEVICE, "no HIP device is available");
+	}
+	if (device < 0 || device >= ndev)
+		return fail(XSB_ERR_NO_DEVICE, "device %d not in [0, %d)", device, ndev);
+	XSB_HIP(hipSetDevice(device));
+	Device *d = new Device();
+	*out = d;
+	d->device = device;
+	hipDeviceProp_t prop;
+	XSB_HIP(hipGetDeviceProperties(&prop, device));
+	d->cu_count = prop.multiProcessorCount;
+	snprintf(d->arch, sizeof(d->arch), "%s", prop.gcnArchName);
+	XSB_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+	for (auto &e : d->ev)
+		XSB_HIP(hipEventCreate(&e));
+
+	const int64_t n_iso = t->n_isotopes, n_gp = t->n_gridpoints;
+	if ((rc = upload_array(&d->d_grid, t->nuclide_grid, n_iso * n_gp * kPointDoubles, &d->bytes_fp64)))
+		return rc;
+	if ((rc = upload_array(&d->d_num_nucs, t->num_nucs, kMaterials, nullptr)))
+		return rc;
+	if ((rc = upload_array(&d->d_mats, t->mats, (int64_t)kMaterials * t->max_num_nucs, nullptr)))
+		return rc;
+	if ((rc = upload_array(&d->d_concs, t->concs, (int64_t)kMaterials * t->max_num_nucs, nullptr)))
+		return rc;
+	if (t->grid_type == XSB_UNIONIZED) {
+		if ((rc = upload_array(&d->d_ueg, t->unionized_energy_array, t->length_unionized_energy_array,
+		                       &d->bytes_union)))
+			return rc;
+	}
+	if (t->grid_type != XSB_NUCLIDE) {
+		if ((rc = upload_array(&d->d_index, t->index_grid, t->length_index_grid, &d->bytes_index)))
+			return rc;
+	}
+	if (hipMalloc((void **)&d->d_acc, 64 * sizeof(unsigned long long)) != hipSuccess) {
+		(void)hipGetLastError();
+		d->d_acc = nullptr;
+		return fail(XSB_ERR_OOM, "accumulator allocation failed");
+	}
+	XSB_HIP(hipMemset(d->d_acc, 0, 64 * sizeof(unsigned long long)));
+
+	// integrity checks that protect the kernels' memory accesses
+	unsigned long long bad = 0;
+	if (d->d_index) {
+		const int32_t hi = (t->grid_type == XSB_HASH) ? (int32_t)(n_gp - 2) : (int32_t)(n_gp - 1);
+		hipLaunchKernelGGL(k_index_range, dim3(4096), dim3(256), 0, d->stream, d->d_index,
+		                   (int64_t)t->length_index_grid, hi, d->d_acc + 60);
+		XSB_HIP(hipGetLastError());
+	}
+	if (d->d_ueg) {
+		hipLaunchKernelGGL(k_sorted_check, dim3(2048), dim3(256), 0, d->stream, d->d_ueg,
+		                   (int64_t)t->length_unionized_energy_array, d->d_acc + 61);
+		XSB_HIP(hipGetLastError());
+	}
+	unsigned long long chk[2] = {0, 0};
+	XSB_HIP(hipMemcpyAsync(chk, d->d_acc + 60, sizeof(chk), hipMemcpyDeviceToHost, d->stream));
+	XSB_HIP(hipStreamSynchronize(d->stream));
+	bad = chk[0];
+	if (bad)
+		return fail(XSB_ERR_INVALID, "index grid holds %llu entries outside [0, %lld]", bad,
+		            (long long)(t->grid_type == XSB_HASH ? n_gp - 2 : n_gp - 1));
+	if (chk[1])
+		return fail(XSB_ERR_INVALID, "unionized energy array is not sorted (%llu descents)", chk[1]);
+
+	Tables &T = d->T;
+	T.ueg = d->d_ueg;
+	T.index_grid = d->d_index;
+	T.grid = d->d_grid;
+	T.num_nucs = d->d_num_nucs;
+	T.mats = d->d_mats;
+	T.concs = d->d_concs;
+	T.n_union = (t->grid_type == XSB_UNIONIZED) ? t->length_unionized_energy_array : 0;
+	T.n_iso = (int32_t)n_iso;
+	T.n_gp = (int32_t)n_gp;
+	T.hash_bins = t->hash_bins;
+	T.max_num_nucs = t->max_num_nucs;
+	T.grid_type = t->grid_type;
+	T.hash_du = (t->hash_bins > 0) ? 1.0 / t->hash_bins : 0.0;
+	pick_thresholds(T.thr);
+	for (int i = 1; i < kMaterials; i++)
+		if (!(T.thr[i] > T.thr[i - 1]))
+			return fail(XSB_ERR_INVALID, "material thresholds are not increasing");
+	for (int m = 0; m < kMaterials; m++)
+		d->num_nucs_host[m] = t->num_nucs[m];
+
+	if ((rc = build_compact(d, t, flags)))
+		return rc;
+	T.xs32 = reinterpret_cast<const float *>(d->d_xs32);
+	T.xs16 = d->d_xs16;
+	T.xs16_scale = d->d_xs16_scale;
+	T.concs32 = d->d_concs32;
+	return XSB_OK;
+}
+
+static int check_handle(void *h, Device **d)
+{
+	if (!h)
+		return fail(XSB_ERR_INVALID, "handle is NULL");
+	*d = static_cast<Device *>(h);
+	XSB_HIP(hipSetDevice((*d)->device));
+	return XSB_OK;
+}
+
+static float elapsed(hipEvent_t a, hipEvent_t b)
+{
+	float ms = 0.f;
+	if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+		(void)hipGetLastError();
+		return 0.f;
+	}
+	return ms;
+}
+
+static int precision_ok(Device *d, int prec)
+{
+	if (prec == XSB_XS_FP64)
+		return XSB_OK;
+	if (prec == XSB_XS_FP32)
+		return d->d_xs32 ? XSB_OK
+		                 : fail(XSB_ERR_UNSUPPORTED, "FP32 tables were not built (XSB_BUILD_FP32)");
+	if (prec == XSB_XS_BF16)
+		return d->d_xs16 ? XSB_OK
+		                 : fail(XSB_ERR_UNSUPPORTED, "BF16 tables were not built (XSB_BUILD_BF16)");
+	return fail(XSB_ERR_INVALID, "xs_precision %d is not 0, 1 or 2", prec);
+}
+
+// Per-lookup detail buffers on the device, copied back after the run.
+struct DetailBufs {
+	DetailOut dev{};
+	bool any = false;
+};
+
+static int detail_alloc(Device *d, int64_t n, const xsb_event_args *a, DetailBufs *b)
+{
+	b->any = a->out_energy || a->out_mat || a->out_index || a->out_macro || a->out_argmax;
+	if (!b->any)
+		return XSB_OK;
+	const size_t per = 8 + 4 + 8 + 40 + 4 + 4;
+	void *base = nullptr;
+	int rc = scratch_reserve(d, 3, (size_t)n * per + 4096, &base);
+	if (rc)
+		return rc;
+	char *p = (char *)base;
+	b->dev.macro = a->out_macro ? (double *)p : nullptr;
+	p += (size_t)n * 40;
+	b->dev.energy = a->out_energy ? (double *)p : nullptr;
+	p += (size_t)n * 8;
+	b->dev.index = a->out_index ? (int64_t *)p : nullptr;
+	p += (size_t)n * 8;
+	b->dev.mat = a->out_mat ? (int32_t *)p : nullptr;
+	p += (size_t)n * 4;
+	b->dev.argmax = a->out_argmax ? (int32_t *)p : nullptr;
+	p += (size_t)n * 4;
+	b->dev.flag = nullptr;
+	return XSB_OK;
+}
+
+static int detail_copy_back(Device *d, int64_t n, const xsb_event_args *a, const DetailBufs *b)
+{
+	if (!b->any)
+		return XSB_OK;
+	hipStream_t s = d->stream;
+	if (a->out_macro)
+		XSB_HIP(hipMemcpyAsync(a->out_macro, b->dev.macro, n * 40, hipMemcpyDeviceToHost, s));
+	if (a->out_energy)
+		XSB_HIP(hipMemcpyAsync(a->out_energy, b->dev.energy, n * 8, hipMemcpyDeviceToHost, s));
+	if (a->out_index)
+		XSB_HIP(hipMemcpyAsync(a->out_index, b->dev.index, n * 8, hipMemcpyDeviceToHost, s));
+	if (a->out_mat)
+		XSB_HIP(hipMemcpyAsync(a->out_mat, b->dev.mat, n * 4, hipMemcpyDeviceToHost, s));
+	if (a->out_argmax)
+		XSB_HIP(hipMemcpyAsync(a->out_argmax, b->dev.argmax, n * 4, hipMemcpyDeviceToHost, s));
+	XSB_HIP(hipStreamSynchronize(s));
+	return XSB_OK;
+}
+
+static int event_impl(Device *d, const xsb_event_args *a, xsb_result *r)
+{
+	if (!a || !r)
+		return fail(XSB_ERR_INVALID, "args and result must be non-NULL");
+	if (a->lookups < 0 || a->first_lookup < 0 || a->lookups > ((int64_t)1 << 32) ||
+	    a->first_lookup > ((int64_t)1 << 61))
+		return fail(XSB_ERR_INVALID, "lookups=%lld first_lookup=%lld out of range",
+		            (long long)a->lookups, (long long)a->first_lookup);
+	int rc = precision_ok(d, a->xs_precision);
+	if (rc)
+		return rc;
+	int k = a->kernel_id;
+	if (k == XSB_K_AUTO)
+		k = XSB_K_MAT_E_SORT;
+	if (k < 0 || k > 6)
+		return fail(XSB_ERR_INVALID, "kernel_id %d is not in [-1, 6]", a->kernel_id);
+	if (k == XSB_K_BASELINE && a->xs_precision != XSB_XS_FP64)
+		k = XSB_K_SPLIT;  // the fused kernel is the FP64 baseline only
+	memset(r, 0, sizeof(*r));
+	r->kernel_id = k;
+	const int64_t n = a->lookups;
+	hipStream_t s = d->stream;
+	DetailBufs db;
+	if ((rc = detail_alloc(d, n, a, &db)))
+		return rc;
+	unsigned long long *vsum = d->d_acc, *nfb = d->d_acc + 1;
+	XSB_HIP(hipMemsetAsync(d->d_acc, 0, 2 * sizeof(unsigned long long), s));
+	XSB_HIP(hipEventRecord(d->ev[0], s));
+	if (k == XSB_K_BASELINE) {
+		XSB_HIP(hipEventRecord(d->ev[1], s));
+		if ((rc = launch_event_fused(d, a->first_lookup, n, a->seed, db.any ? &db.dev : nullptr,
+		                             vsum, s)))
+			return rc;
+		XSB_HIP(hipEventRecord(d->ev[2], s));
+	} else if (k == XSB_K_SPLIT) {
+		void *buf = nullptr;
+		if ((rc = scratch_reserve(d, 0, (size_t)n * 12 + 256, &buf)))
+			return rc;
+		double *E = (double *)buf;
+		int32_t *M = (int32_t *)((char *)buf + (size_t)n * 8);
+		if ((rc = launch_sample(d, a->first_lookup, n, a->seed, E, M, s)))
+			return rc;
+		XSB_HIP(hipEventRecord(d->ev[1], s));
+		if ((rc = launch_lookup_list(d, n, E, M, nullptr, a->xs_precision,
+		                             db.any ? &db.dev : nullptr, vsum, nfb, s)))
+			return rc;
+		XSB_HIP(hipEventRecord(d->ev[2], s));
+	} else {
+		if ((rc = run_sorted_event(d, a, db.any ? &db.dev : nullptr, r, vsum, nfb)))
+			return rc;
+	}
+	XSB_HIP(hipEventRecord(d->ev[3], s));
+	unsigned long long acc[2];
+	XSB_HIP(hipMemcpyAsync(acc, d->d_acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+	XSB_HIP(hipStreamSynchronize(s));
+	r->vsum = acc[0];
+	r->n_fallback = (int64_t)acc[1];
+	r->ms_total = elapsed(d->ev[0], d->ev[3]);
+	r->ms_sample = elapsed(d->ev[0], d->ev[1]);
+	r->ms_lookup = elapsed(d->ev[1], d->ev[2]);
+	return detail_copy_back(d, n, a, &db);
+}
+
+}  // namespace xsb
+
+using namespace xsb;
+
+extern "C" {
+
+int xsb_hip_upload(int device, const xsb_tables *t, uint32_t flags, void **handle)
+{
+	if (!handle)
+		return fail(XSB_ERR_INVALID, "handle output pointer is NULL");
+	*handle = nullptr;
+	Device *d = nullptr;
+	int rc = upload(device, t, flags, &d);
+	if (rc) {
+		release(d);
+		return rc;
+	}
+	*handle = d;
+	return XSB_OK;
+}
+
+int xsb_hip_release(void *handle)
+{
+	if (!handle)
+		return XSB_OK;
+	release(static_cast<Device *>(handle));
+	return XSB_OK;
+}
+
+int xsb_hip_event(void *handle, const xsb_event_args *args, xsb_result *res)
+{
+	Device *d;
+	int rc = check_handle(handle, &d);
+	return rc ? rc : event_impl(d, args, res);
+}
+
+int xsb_hip_history(void *handle, const xsb_history_args *a, xsb_result *r)
+{
+	Device *d;
+	int rc = check_handle(handle, &d);
+	if (rc)
+		return rc;
+	if (!a || !r)
+		return fail(XSB_ERR_INVALID, "args and result must be non-NULL");
+	if (a->particles < 0 || a->particles > INT32_MAX || a->lookups_per_particle < 0)
+		return fail(XSB_ERR_INVALID, "particles=%lld lookups=%d out of range",
+		            (long long)a->particles, a->lookups_per_particle);
+	if ((rc = precision_ok(d, a->xs_precision)))
+		return rc;
+	memset(r, 0, sizeof(*r));
+	r->kernel_id = 0;
+	hipStream_t s = d->stream;
+	const int64_t n = a->particles;
+	uint64_t *dsum = nullptr, *dseed = nullptr;
+	if (a->out_particle_sum || a->out_final_seed) {
+		void *buf = nullptr;
+		if ((rc = scratch_reserve(d, 3, (size_t)n * 16 + 256, &buf)))
+			return rc;
+		dsum = (uint64_t *)buf;
+		dseed = dsum + n;
+	}
+	XSB_HIP(hipMemsetAsync(d->d_acc, 0, 2 * sizeof(unsigned long long), s));
+	XSB_HIP(hipEventRecord(d->ev[0], s));
+	if ((rc = launch_history(d, n, a->lookups_per_particle, a->seed, a->xs_precision, dsum, dseed,
+	                         d->d_acc, d->d_acc + 1, s)))
+		return rc;
+	XSB_HIP(hipEventRecord(d->ev[3], s));
+	unsigned long long acc[2];
+	XSB_HIP(hipMemcpyAsync(acc, d->d_acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+	if (a->out_particle_sum)
+		XSB_HIP(hipMemcpyAsync(a->out_particle_sum, dsum, n * 8, hipMemcpyDeviceToHost, s));
+	if (a->out_final_seed)
+		XSB_HIP(hipMemcpyAsync(a->out_final_seed, dseed, n * 8, hipMemcpyDeviceToHost, s));
+	XSB_HIP(hipStreamSynchronize(s));
+	r->vsum = acc[0];
+	r->n_fallback = (int64_t)acc[1];
+	r->ms_total = r->ms_lookup = elapsed(d->ev[0], d->ev[3]);
+	return XSB_OK;
+}
+
+int xsb_hip_lookup(void *handle, int64_t n, const double *d_energy, const int32_t *d_mat,
+                   int32_t xs_precision, double *d_macro, int32_t *d_argmax,
+                   unsigned long long *d_vsum, void *stream)
+{
+	Device *d;
+	int rc = check_handle(handle, &d);
+	if (rc)
+		return rc;
+	if (n < 0 || n > ((int64_t)1 << 32) || (n > 0 && (!d_energy || !d_mat)))
+		return fail(XSB_ERR_INVALID, "n=%lld or sample pointers invalid", (long long)n);
+	if ((rc = precision_ok(d, xs_precision)))
+		return rc;
+	hipStream_t s = stream ? (hipStream_t)stream : d->stream;
+	DetailOut det{};
+	det.macro = d_macro;
+	det.argmax = d_argmax;
+	const bool detail = d_macro || d_argmax;
+	return launch_lookup_list(d, n, d_energy, d_mat, nullptr, xs_precision, detail ? &det : nullptr,
+	                          d_vsum ? d_vsum : d->d_acc + 40, d->d_acc + 41, s);
+}
+
+int xsb_hip_balg(void *handle, const xsb_event_args *a, double *bytes_per_lookup)
+{
+	Device *d;
+	int rc = check_handle(handle, &d);
+	if (rc)
+		return rc;
+	if (!a || !bytes_per_lookup || a->lookups < 0)
+		return fail(XSB_ERR_INVALID, "bad arguments");
+	return launch_balg(d, a->first_lookup, a->lookups, a->seed, a->xs_precision, bytes_per_lookup,
+	                   d->stream);
+}
+
+int xsb_hip_copy_probe(void *handle, int64_t bytes, int32_t reps, double *ms_per_copy)
+{
+	Device *d;
+	int rc = check_handle(handle, &d);
+	if (rc)
+		return rc;
+	if (bytes < 16 || reps < 1 || !ms_per_copy)
+		return fail(XSB_ERR_INVALID, "bytes >= 16, reps >= 1 and an output pointer are required");
+	bytes &= ~(int64_t)15;
+	void *buf = nullptr;
+	if ((rc = scratch_reserve(d, 2, (size_t)bytes * 2, &buf)))
+		return rc;
+	char *src = (char *)buf, *dst = src + bytes;
+	hipStream_t s = d->stream;
+	XSB_HIP(hipMemsetAsync(src, 1, bytes, s));
+	if ((rc = launch_copy(src, dst, bytes, s)))
+		return rc;
+	XSB_HIP(hipEventRecord(d->ev[4], s));
+	for (int i = 0; i < reps; i++)
+		if ((rc = launch_copy(src, dst, bytes, s)))
+			return rc;
+	XSB_HIP(hipEventRecord(d->ev[5], s));
+	XSB_HIP(hipStreamSynchronize(s));
+	*ms_per_copy = elapsed(d->ev[4], d->ev[5]) / reps;
+	return XSB_OK;
+}
+
+int xsb_hip_info(void *handle, xsb_info *info)
+{
+	Device *d;
+	int rc = check_handle(handle, &d);
+	if (rc)
+		return rc;
+	if (!info)
+		return fail(XSB_ERR_INVALID, "info is NULL");
+	memset(info, 0, sizeof(*info));
+	info->device = d->device;
+	info->grid_type = d->T.grid_type;
+	info->n_isotopes = d->T.n_iso;
+	info->n_gridpoints = d->T.n_gp;
+	info->n_union = d->T.n_union;
+	info->bytes_fp64 = d->bytes_fp64;
+	info->bytes_fp32 = d->bytes_fp32;
+	info->bytes_bf16 = d->bytes_bf16;
+	info->bytes_index = d->bytes_index;
+	info->bytes_union = d->bytes_union;
+	info->cu_count = d->cu_count;
+	info->xcd_count = d->xcd_count;
+	snprintf(info->arch, sizeof(info->arch), "%s", d->arch);
+	return XSB_OK;
+}
+
+const char *xsb_hip_strerror(int code)
+{
+	switch (code) {
+	case XSB_OK: return "success";
+	case XSB_ERR_INVALID: return "invalid argument";
+	case XSB_ERR_NO_DEVICE: return "no usable HIP device";
+	case XSB_ERR_OOM: return "device out of memory";
+	case XSB_ERR_HIP: return "HIP runtime error";
+	case XSB_ERR_UNSUPPORTED: return "unsupported option";
+	default: return "unknown error";
+	}
+}
+
+const char *xsb_hip_last_error(void)
+{
+	return g_msg;
+}
+
+}  // extern "C"

@@ -1,0 +1,257 @@
+// xsb_device.h -- device-side building blocks of the lookup path (gfx950).
+//
+// Everything here is compiled with -ffp-contract=off: each FP64 operation is
+// the single IEEE operation the reference performs, in the reference's order,
+// so the FP64 path reproduces openmp-threading's results bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xsb {
+
+constexpr uint64_t kLcgA = 2806196910506780709ULL;
+constexpr uint64_t kLcgMask = 0x7FFFFFFFFFFFFFFFULL;  // mod 2^63
+constexpr int kMaterials = 12;
+constexpr int kPointDoubles = 6;  // energy + 5 XS channels (NuclideGridPoint)
+
+enum GridType : int { kUnionized = 0, kNuclide = 1, kHash = 2 };
+
+// Device view of the tables.  Passed by value as a kernel argument.
+struct Tables {
+	const double *__restrict__ ueg;         // unionized energies [n_union]
+	const int32_t *__restrict__ index_grid; // [n_union or hash_bins][n_iso]
+	const double *__restrict__ grid;        // FP64 AoS, 6 doubles per point
+	const double *__restrict__ egrid;       // FP64 energies only [n_iso*n_gp] (mixed tables)
+	const float *__restrict__ xs32;         // FP32 XS, 8 floats per point (5 used)
+	const uint16_t *__restrict__ xs16;      // BF16 XS, 8 halves per point (5 used)
+	const float *__restrict__ xs16_scale;   // per nuclide x channel scale (5 per nuclide, 8 stride)
+	const int32_t *__restrict__ num_nucs;   // [12]
+	const int32_t *__restrict__ mats;       // [12][max_num_nucs]
+	const double *__restrict__ concs;       // [12][max_num_nucs]
+	const float *__restrict__ concs32;      // FP32 copy of concs
+	int64_t n_union;
+	int32_t n_iso;
+	int32_t n_gp;
+	int32_t hash_bins;
+	int32_t max_num_nucs;
+	int32_t grid_type;
+	int32_t pad0;
+	double thr[kMaterials];  // pick_mat thresholds, reverse-order partial sums
+	double hash_du;          // 1.0 / hash_bins, as computed by the reference
+};
+
+// ---------------------------------------------------------------------------
+// Sampling (openmp-threading/Simulation.c:426-499)
+// ---------------------------------------------------------------------------
+
+// LCG_random_double: s = (a s + 1) mod 2^63, value s / 2^63.  The u64 -> f64
+// conversion is correctly rounded on both sides (hi*2^32 + lo in one rounded
+// add here, cvtsi2sd on x86), and the scaling by 2^-63 is exact.
+__device__ __forceinline__ double lcg_double(uint64_t &s)
+{
+	s = (kLcgA * s + 1ULL) & kLcgMask;
+	return (double)s * 0x1p-63;
+}
+
+// fast_forward_LCG: O(log n) skip-ahead by square-and-multiply.
+__device__ __forceinline__ uint64_t ff_lcg(uint64_t seed, uint64_t n)
+{
+	uint64_t a = kLcgA, c = 1ULL, an = 1ULL, cn = 0ULL;
+	n &= kLcgMask;
+	while (n) {
+		if (n & 1ULL) {
+			an *= a;
+			cn = cn * a + c;
+		}
+		c *= (a + 1ULL);
+		a *= a;
+		n >>= 1;
+	}
+	return (an * seed + cn) & kLcgMask;
+}
+
+// pick_mat: the reference returns the first i with roll < thr[i], where
+// thr[i] is its reverse-order partial sum (thr[0] = 0), else 0.  The host
+// checks that thr is strictly increasing, so that first i equals the number
+// of thresholds <= roll.
+__device__ __forceinline__ int pick_mat(uint64_t &s, const double *thr)
+{
+	double roll = lcg_double(s);
+	int cnt = 0;
+#pragma unroll
+	for (int i = 0; i < kMaterials; i++)
+		cnt += (thr[i] <= roll) ? 1 : 0;
+	return cnt < kMaterials ? cnt : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Searches (openmp-threading/Simulation.c:380-423).  Both return the unique
+// clamp(upper_bound(q) - 1, low, high - 1); any exact bisection gives it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t grid_search(int32_t n, double q, const double *__restrict__ A)
+{
+	int32_t lo = 0, hi = n - 1;
+	while (hi - lo > 1) {
+		int32_t mid = lo + ((hi - lo) >> 1);
+		if (A[mid] > q)
+			hi = mid;
+		else
+			lo = mid;
+	}
+	return lo;
+}
+
+// bisection on energies stored with a stride (in doubles)
+template <int STRIDE>
+__device__ __forceinline__ int32_t grid_search_nuclide(double q, const double *__restrict__ E,
+                                                       int32_t low, int32_t high)
+{
+	int32_t lo = low, hi = high;
+	while (hi - lo > 1) {
+		int32_t mid = lo + ((hi - lo) >> 1);
+		if (E[(int64_t)mid * STRIDE] > q)
+			hi = mid;
+		else
+			lo = mid;
+	}
+	return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Per-nuclide bracket (openmp-threading/Simulation.c:241-302): the lower
+// point index `lower` such that points lower and lower+1 are interpolated.
+// E_row points at the nuclide's first energy word, stored with stride STRIDE.
+// ---------------------------------------------------------------------------
+template <int GRID, int STRIDE>
+__device__ __forceinline__ int32_t nuclide_lower(const Tables &T, const double *__restrict__ E_row,
+                                                 int nuc, int64_t idx, double p)
+{
+	const int32_t n_gp = T.n_gp;
+	int32_t lower;
+	if (GRID == kUnionized) {
+		lower = T.index_grid[idx * T.n_iso + nuc];
+	} else if (GRID == kNuclide) {
+		lower = grid_search_nuclide<STRIDE>(p, E_row, 0, n_gp - 1);
+	} else {
+		const int32_t u_low = T.index_grid[idx * T.n_iso + nuc];
+		const int32_t u_high = (idx == T.hash_bins - 1)
+		                           ? n_gp - 1
+		                           : T.index_grid[(idx + 1) * T.n_iso + nuc] + 1;
+		const double e_low = E_row[(int64_t)u_low * STRIDE];
+		const double e_high = E_row[(int64_t)u_high * STRIDE];
+		if (p <= e_low)
+			lower = 0;
+		else if (p >= e_high)
+			lower = n_gp - 1;
+		else
+			lower = grid_search_nuclide<STRIDE>(p, E_row, u_low, u_high);
+	}
+	if (lower == n_gp - 1)
+		lower--;
+	return lower;
+}
+
+// Grid-level index for a lookup: unionized-grid bracket or hash bin
+// (openmp-threading/Simulation.c:346-352).  The hash bin keeps the
+// reference's division p / (1.0 / bins); it is clamped to the last bin so
+// that p == 1.0 (which the reference would read past its table with) stays
+// in bounds.
+template <int GRID>
+__device__ __forceinline__ int64_t lookup_index(const Tables &T, double p)
+{
+	if (GRID == kUnionized)
+		return grid_search((int32_t)T.n_union, p, T.ueg);
+	if (GRID == kHash) {
+		int32_t b = (int32_t)(p / T.hash_du);
+		return b < T.hash_bins ? b : T.hash_bins - 1;
+	}
+	return -1;
+}
+
+// FP64 interpolation of the 5 channels between points lo and hi
+// (openmp-threading/Simulation.c:306-322), accumulated into macro in the
+// reference's order (Simulation.c:372-373).
+__device__ __forceinline__ void accumulate_fp64(const double *__restrict__ pt, double p, double conc,
+                                                double macro[5])
+{
+	const double2 *q = reinterpret_cast<const double2 *>(pt);
+	const double2 a = q[0], b = q[1], c = q[2];  // lo: E t | el ab | fi nf
+	const double2 d = q[3], e = q[4], f = q[5];  // hi: E t | el ab | fi nf
+	const double fr = (d.x - p) / (d.x - a.x);
+	macro[0] += (d.y - fr * (d.y - a.y)) * conc;
+	macro[1] += (e.x - fr * (e.x - b.x)) * conc;
+	macro[2] += (e.y - fr * (e.y - b.y)) * conc;
+	macro[3] += (f.x - fr * (f.x - c.x)) * conc;
+	macro[4] += (f.y - fr * (f.y - c.y)) * conc;
+}
+
+// calculate_macro_xs (openmp-threading/Simulation.c:326-375), FP64.
+template <int GRID>
+__device__ __forceinline__ void macro_xs_fp64(const Tables &T, double p, int mat, int64_t idx,
+                                              double macro[5])
+{
+#pragma unroll
+	for (int c = 0; c < 5; c++)
+		macro[c] = 0.0;
+	const int nn = T.num_nucs[mat];
+	const int32_t *__restrict__ ml = T.mats + mat * T.max_num_nucs;
+	const double *__restrict__ cl = T.concs + mat * T.max_num_nucs;
+	for (int j = 0; j < nn; j++) {
+		const int nuc = ml[j];
+		const double conc = cl[j];
+		const double *row = T.grid + (int64_t)nuc * T.n_gp * kPointDoubles;
+		const int32_t lower = nuclide_lower<GRID, kPointDoubles>(T, row, nuc, idx, p);
+		accumulate_fp64(row + (int64_t)lower * kPointDoubles, p, conc, macro);
+	}
+}
+
+// argmax with max = -1.0 and strict '>' (Simulation.c:100-110)
+__device__ __forceinline__ int argmax5(const double m[5])
+{
+	double mx = -1.0;
+	int am = 0;
+#pragma unroll
+	for (int j = 0; j < 5; j++) {
+		if (m[j] > mx) {
+			mx = m[j];
+			am = j;
+		}
+	}
+	return am;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+{
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1)
+		v += __shfl_xor(v, off, 64);
+	return v;
+}
+
+// Sum v over the block and add it to *out with one device-scope atomic.
+template <int BLOCK>
+__device__ __forceinline__ void block_add_u64(uint32_t v, unsigned long long *out)
+{
+	static_assert(BLOCK % 64 == 0, "block must be whole waves");
+	__shared__ uint32_t part[BLOCK / 64];
+	v = wave_sum_u32(v);
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	__syncthreads();  // a previous call's reader (thread 0) is done with part[]
+	if (lane == 0)
+		part[wave] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		unsigned long long s = 0;
+#pragma unroll
+		for (int w = 0; w < BLOCK / 64; w++)
+			s += part[w];
+		if (s)
+			atomicAdd(out, s);
+	}
+}
+
+}  // namespace xsb

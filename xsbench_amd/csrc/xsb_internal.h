@@ -1,0 +1,90 @@
+// xsb_internal.h -- host-side internals shared by the HIP translation units
+// of libxsb_hip.so (device handle, error plumbing, kernel launchers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "xsb_device.h"
+#include "xsb_hip.h"
+
+namespace xsb {
+
+// Records a message for xsb_hip_last_error() and returns `code`.
+int fail(int code, const char *fmt, ...);
+
+#define XSB_HIP(call)                                                                  \
+	do {                                                                               \
+		hipError_t e_ = (call);                                                        \
+		if (e_ != hipSuccess)                                                          \
+			return ::xsb::fail(XSB_ERR_HIP, "%s failed: %s (%s:%d)", #call,            \
+			                   hipGetErrorString(e_), __FILE__, __LINE__);             \
+	} while (0)
+
+// Growable device scratch owned by a handle.
+struct Scratch {
+	void *ptr = nullptr;
+	size_t bytes = 0;
+};
+
+struct Device {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	Tables T{};
+	int cu_count = 0;
+	int xcd_count = 8;
+	char arch[32] = {0};
+	// owned allocations
+	double *d_ueg = nullptr;
+	int32_t *d_index = nullptr;
+	double *d_grid = nullptr;
+	double *d_egrid = nullptr;
+	float *d_xs32 = nullptr;
+	uint16_t *d_xs16 = nullptr;
+	float *d_xs16_scale = nullptr;
+	int32_t *d_num_nucs = nullptr;
+	int32_t *d_mats = nullptr;
+	double *d_concs = nullptr;
+	float *d_concs32 = nullptr;
+	unsigned long long *d_acc = nullptr;  // 64 accumulators
+	int64_t bytes_fp64 = 0, bytes_fp32 = 0, bytes_bf16 = 0, bytes_index = 0, bytes_union = 0;
+	int32_t num_nucs_host[kMaterials] = {0};
+	Scratch scratch[4];
+	hipEvent_t ev[16] = {};
+};
+
+// Grows scratch slot `slot` of `d` to at least `bytes`.
+int scratch_reserve(Device *d, int slot, size_t bytes, void **out);
+
+// Per-lookup optional outputs of the lookup kernels (device pointers).
+struct DetailOut {
+	double *energy;
+	int32_t *mat;
+	int64_t *index;
+	double *macro;
+	int32_t *argmax;
+	int32_t *flag;  // mixed precision: 1 where the lookup was re-evaluated in FP64
+};
+
+// ---- launchers (xsb_lookup.hip) ----
+int launch_event_fused(Device *d, int64_t first, int64_t n, uint64_t seed, const DetailOut *det,
+                       unsigned long long *vsum, hipStream_t s);
+int launch_sample(Device *d, int64_t first, int64_t n, uint64_t seed, double *E, int32_t *mat,
+                  hipStream_t s);
+int launch_lookup_list(Device *d, int64_t n, const double *E, const int32_t *mat,
+                       const uint32_t *perm, int precision, const DetailOut *det,
+                       unsigned long long *vsum, unsigned long long *nfallback, hipStream_t s);
+int launch_history(Device *d, int64_t particles, int lookups, uint64_t seed, int precision,
+                   uint64_t *out_sum, uint64_t *out_seed, unsigned long long *vsum,
+                   unsigned long long *nfallback, hipStream_t s);
+int launch_balg(Device *d, int64_t first, int64_t n, uint64_t seed, int precision, double *out_sum,
+                hipStream_t s);
+int launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);
+
+// ---- sorted schedules (xsb_sched.hip) ----
+int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, xsb_result *r,
+                     unsigned long long *vsum, unsigned long long *nfb);
+
+}  // namespace xsb

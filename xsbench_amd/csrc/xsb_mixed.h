@@ -1,0 +1,158 @@
+// xsb_mixed.h -- mixed-precision macroscopic XS (FP32 or BF16 XS tables,
+// FP64 energies, FP32 Kahan accumulation) with an FP64 fallback that keeps
+// every decision identical to the FP64 reference path.
+//
+// Table records (built on the device from the FP64 AoS at upload):
+//   FP32: 32 B = { double E; float xs[5]; float pad; }
+//   BF16: 24 B = { double E; bf16 xs[5]; u16 pad[3]; }  xs scaled by a power
+//         of two per nuclide, s_n >= max |xs| of that nuclide
+// Energies stay FP64 so that brackets and the interpolation fraction are the
+// reference's exactly; only the 5 channel values are rounded.
+//
+// Error bound (per lookup, shared by the 5 channels).  With u the unit
+// roundoff of the table format (2^-24 FP32, 2^-8 BF16), u32 = 2^-24, M_j an
+// upper bound of |xs| for nuclide j and f_j the FP64 fraction, every term
+// conc_j * xs_j of the reference differs from ours by at most
+// conc_j M_j (1 + 2|f_j|) (u + 7 u32 + 5 u64), and the Kahan sum adds at most
+// 2 u32 |S| + n u32^2 sum|t|; the reference's own ordered FP64 sum differs from
+// the exact sum by (n-1) u64 sum|r|.  So |S_c - R_c| <= B with
+//   B = 1.01 * A * (u + 12 u32),   A = sum_j conc_j M_j (1 + 2|f_j|)
+// (the 1.01 absorbs the FP32 evaluation of A itself and the rounding of
+// conc_j M_j, for n < 10^5).  A lookup whose argmax gap is <= 2B, or (history
+// mode) with some |S_c - 1| <= B, is recomputed in FP64.
+#pragma once
+
+#include "xsb_device.h"
+
+namespace xsb {
+
+constexpr int kRec32Doubles = 4;  // 32 B record
+constexpr int kRec16Doubles = 3;  // 24 B record
+
+// Per (material, position) constants for the mixed kernels, [12][max_num_nucs]
+// float2 {c_eff, c_bound}: c_eff = conc * scale (scale = 1 for FP32),
+// c_bound = conc * M (rounded up) for the error bound.  Stored in T.concs32
+// as two consecutive [12][max] float planes per precision:
+//   plane 0/1: FP32 c_eff / c_bound, plane 2/3: BF16 c_eff / c_bound.
+
+__device__ __forceinline__ float bf16_to_f32(uint32_t h)
+{
+	return __uint_as_float(h << 16);
+}
+
+template <int PREC>
+__device__ __forceinline__ const double *mixed_row(const Tables &T, int nuc)
+{
+	if (PREC == XSB_XS_FP32)
+		return reinterpret_cast<const double *>(T.xs32) + (int64_t)nuc * T.n_gp * kRec32Doubles;
+	return reinterpret_cast<const double *>(T.xs16) + (int64_t)nuc * T.n_gp * kRec16Doubles;
+}
+
+// Load the 5 channel values of a record.
+template <int PREC>
+__device__ __forceinline__ void load_xs(const double *rec, float x[5])
+{
+	if (PREC == XSB_XS_FP32) {
+		const float4 a = *reinterpret_cast<const float4 *>(rec + 1);  // xs0..xs3
+		const float b = *reinterpret_cast<const float *>(rec + 3);    // xs4
+		x[0] = a.x;
+		x[1] = a.y;
+		x[2] = a.z;
+		x[3] = a.w;
+		x[4] = b;
+	} else {
+		const uint2 w = *reinterpret_cast<const uint2 *>(rec + 1);          // xs0..xs3
+		const uint32_t w2 = *reinterpret_cast<const uint32_t *>(rec + 2);   // xs4 | pad
+		x[0] = bf16_to_f32(w.x & 0xFFFFu);
+		x[1] = bf16_to_f32(w.x >> 16);
+		x[2] = bf16_to_f32(w.y & 0xFFFFu);
+		x[3] = bf16_to_f32(w.y >> 16);
+		x[4] = bf16_to_f32(w2 & 0xFFFFu);
+	}
+}
+
+__device__ __forceinline__ void kahan_add(float &sum, float &comp, float t)
+{
+	const float y = t - comp;
+	const float s = sum + y;
+	comp = (s - sum) - y;
+	sum = s;
+}
+
+// Returns the argmax the FP64 reference would return; macro[] receives the
+// mixed-precision sums (or the FP64 sums when the lookup fell back, which
+// HISTORY mode needs for its > 1.0 test).  *fell_back is set to 1 on fallback.
+template <int GRID, int PREC, bool HISTORY = false>
+__device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat, int64_t idx,
+                                              double macro[5], uint32_t *fell_back)
+{
+	constexpr int REC = (PREC == XSB_XS_FP32) ? kRec32Doubles : kRec16Doubles;
+	constexpr float kU = (PREC == XSB_XS_FP32) ? 0x1p-24f : 0x1p-8f;
+	const int nn = T.num_nucs[mat];
+	const int plane = T.max_num_nucs * kMaterials;
+	const float *__restrict__ ceff = T.concs32 + (PREC == XSB_XS_FP32 ? 0 : 2) * plane +
+	                                 mat * T.max_num_nucs;
+	const float *__restrict__ cbnd = ceff + plane;
+	const int32_t *__restrict__ ml = T.mats + mat * T.max_num_nucs;
+	float sum[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+	float A = 0.f;
+	for (int j = 0; j < nn; j++) {
+		const int nuc = ml[j];
+		const double *row = mixed_row<PREC>(T, nuc);
+		const int32_t lower = nuclide_lower<GRID, REC>(T, row, nuc, idx, p);
+		const double *lo = row + (int64_t)lower * REC;
+		const double *hi = lo + REC;
+		const double e_lo = lo[0], e_hi = hi[0];
+		float xl[5], xh[5];
+		load_xs<PREC>(lo, xl);
+		load_xs<PREC>(hi, xh);
+		const double fr64 = (e_hi - p) / (e_hi - e_lo);
+		const float fr = (float)fr64;
+		const float c = ceff[j];
+#pragma unroll
+		for (int k = 0; k < 5; k++) {
+			const float x = xh[k] - fr * (xh[k] - xl[k]);
+			kahan_add(sum[k], comp[k], x * c);
+		}
+		A += cbnd[j] * (1.0f + 2.0f * fabsf(fr));
+	}
+	// argmax of the mixed sums (max = -1.0, strict '>', as the reference)
+	double s[5];
+#pragma unroll
+	for (int k = 0; k < 5; k++)
+		s[k] = (double)sum[k];
+	const int am = argmax5(s);
+	const double B = 1.01 * (double)A * ((double)kU + 12.0 * 0x1p-24);
+	bool ambiguous = false;
+#pragma unroll
+	for (int k = 0; k < 5; k++) {
+		if (k != am && !(s[am] - s[k] > 2.0 * B))
+			ambiguous = true;
+		if (HISTORY && fabs(s[k] - 1.0) <= B)
+			ambiguous = true;
+	}
+	// the reference's argmax needs max > -1.0 for a non-zero index; every
+	// sum is >= -B here only if the XS are non-negative, so treat a sum
+	// within B of -1.0 as ambiguous too
+	if (s[am] <= -1.0 + B)
+		ambiguous = true;
+#pragma unroll
+	for (int k = 0; k < 5; k++)
+		macro[k] = s[k];
+	if (!ambiguous) {
+		*fell_back = 0;
+		return am;
+	}
+	*fell_back = 1;
+	double m64[5];
+	macro_xs_fp64<GRID>(T, p, mat, idx, m64);
+	if (HISTORY) {
+#pragma unroll
+		for (int k = 0; k < 5; k++)
+			macro[k] = m64[k];
+	}
+	return argmax5(m64);
+}
+
+}  // namespace xsb

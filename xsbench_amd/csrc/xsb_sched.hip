@@ -1,0 +1,400 @@
+// xsb_sched.hip -- sorted lookup schedules (the reference's -k 2..6:
+// openmp-threading/Simulation.c:698-871, cuda/Simulation.cu:514-1099).
+//
+// The samples are never materialised: a key kernel derives each lookup's
+// (material, energy) from its LCG state and emits a sort key plus the lookup
+// id; an LSD radix sort (8-bit digits, stable per-wave ranking with ballots)
+// orders the ids; the lookup kernel re-derives (E, mat) from the id.  One
+// launch covers all material segments, so no per-material tail waves.
+//
+//   k = 2, 4 : key = material                       (material segments)
+//   k = 3, 5 : key = (material != fuel)             (fuel first)
+//   k = 6    : key = material << 20 | energy bits   (material, then energy)
+#include "xsb_internal.h"
+#include "xsb_mixed.h"
+
+namespace xsb {
+
+constexpr int kSortBlock = 256;
+constexpr int kSortWaves = kSortBlock / 64;
+constexpr int kSortItems = 16;
+constexpr int kTile = kSortBlock * kSortItems;  // 4096 elements per tile
+constexpr int kEnergyBits = 20;
+constexpr int kScanItems = 16;
+constexpr int kScanChunk = 256 * kScanItems;  // 4096 entries per scan block
+
+enum SortMode : int { kModeMat = 0, kModeFuel = 1, kModeMatE = 2 };
+
+__device__ __forceinline__ int tile_pos(int wave, int item, int lane)
+{
+	return wave * 64 * kSortItems + item * 64 + lane;
+}
+
+__device__ __forceinline__ uint32_t sample_key(const Tables &T, uint64_t seed0, int64_t lookup,
+                                               int mode)
+{
+	uint64_t s = ff_lcg(seed0, 2ULL * (uint64_t)lookup);
+	const double p = lcg_double(s);
+	const int mat = pick_mat(s, T.thr);
+	if (mode == kModeFuel)
+		return mat != 0 ? 1u : 0u;
+	if (mode == kModeMat)
+		return (uint32_t)mat;
+	uint32_t e = (uint32_t)(p * (double)(1u << kEnergyBits));  // exact scaling, truncation
+	e = e < (1u << kEnergyBits) ? e : (1u << kEnergyBits) - 1u;
+	return ((uint32_t)mat << kEnergyBits) | e;
+}
+
+// Keys + ids for lookups [first, first+n), and the first pass's per-tile
+// digit histogram hist[digit * ntiles + tile].
+__global__ __launch_bounds__(kSortBlock) void k_keys(Tables T, int64_t first, int64_t n,
+                                                     uint64_t seed0, int mode, uint32_t *keys,
+                                                     uint32_t *vals, uint32_t *hist, int ntiles)
+{
+	__shared__ uint32_t h[256];
+	h[threadIdx.x] = 0;
+	__syncthreads();
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int64_t base = (int64_t)blockIdx.x * kTile;
+	for (int it = 0; it < kSortItems; it++) {
+		const int64_t e = base + tile_pos(wave, it, lane);
+		if (e < n) {
+			const uint32_t k = sample_key(T, seed0, first + e, mode);
+			keys[e] = k;
+			vals[e] = (uint32_t)e;
+			atomicAdd(&h[k & 0xFFu], 1u);
+		}
+	}
+	__syncthreads();
+	hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_hist(const uint32_t *__restrict__ keys, int64_t n,
+                                                     int shift, uint32_t *hist, int ntiles)
+{
+	__shared__ uint32_t h[256];
+	h[threadIdx.x] = 0;
+	__syncthreads();
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int64_t base = (int64_t)blockIdx.x * kTile;
+	for (int it = 0; it < kSortItems; it++) {
+		const int64_t e = base + tile_pos(wave, it, lane);
+		if (e < n)
+			atomicAdd(&h[(keys[e] >> shift) & 0xFFu], 1u);
+	}
+	__syncthreads();
+	hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// ---- exclusive scan of the digit-major histogram (reduce, scan, apply) ----
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane)
+{
+#pragma unroll
+	for (int off = 1; off < 64; off <<= 1) {
+		const uint32_t y = __shfl_up(x, off, 64);
+		if (lane >= off)
+			x += y;
+	}
+	return x;
+}
+
+// exclusive scan of one value per thread over a 256-thread block
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t *total)
+{
+	__shared__ uint32_t wsum[kSortWaves];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const uint32_t incl = wave_incl_scan(x, lane);
+	if (lane == 63)
+		wsum[wave] = incl;
+	__syncthreads();
+	uint32_t off = 0, tot = 0;
+#pragma unroll
+	for (int w = 0; w < kSortWaves; w++) {
+		off += (w < wave) ? wsum[w] : 0u;
+		tot += wsum[w];
+	}
+	__syncthreads();
+	if (total)
+		*total = tot;
+	return off + incl - x;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t *__restrict__ a, int64_t n,
+                                                     uint32_t *part)
+{
+	const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+	uint32_t s = 0;
+	for (int it = 0; it < kScanItems; it++) {
+		const int64_t i = base + (int64_t)it * 256 + threadIdx.x;
+		if (i < n)
+			s += a[i];
+	}
+	uint32_t tot;
+	block_excl_scan(s, &tot);
+	if (threadIdx.x == 0)
+		part[blockIdx.x] = tot;
+}
+
+// single block: exclusive scan of up to 256 * kScanItems partials
+__global__ __launch_bounds__(256) void k_scan_parts(uint32_t *part, int nparts)
+{
+	uint32_t v[kScanItems];
+	uint32_t s = 0;
+	for (int it = 0; it < kScanItems; it++) {
+		const int i = threadIdx.x * kScanItems + it;
+		v[it] = i < nparts ? part[i] : 0u;
+		s += v[it];
+	}
+	uint32_t off = block_excl_scan(s, nullptr);
+	for (int it = 0; it < kScanItems; it++) {
+		const int i = threadIdx.x * kScanItems + it;
+		if (i < nparts)
+			part[i] = off;
+		off += v[it];
+	}
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(uint32_t *a, int64_t n,
+                                                    const uint32_t *__restrict__ part)
+{
+	const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
+	uint32_t v[kScanItems];
+	uint32_t s = 0;
+	for (int it = 0; it < kScanItems; it++) {
+		const int64_t i = base + it;
+		v[it] = i < n ? a[i] : 0u;
+		s += v[it];
+	}
+	uint32_t off = block_excl_scan(s, nullptr) + part[blockIdx.x];
+	for (int it = 0; it < kScanItems; it++) {
+		const int64_t i = base + it;
+		if (i < n)
+			a[i] = off;
+		off += v[it];
+	}
+}
+
+// Stable scatter of one 8-bit digit.  Tile order is (wave, item, lane); each
+// wave ranks its items in that order with ballot-built peer masks and
+// per-wave digit counters in LDS, then waves are offset per digit.
+__global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t *__restrict__ kin,
+                                                        const uint32_t *__restrict__ vin,
+                                                        uint32_t *__restrict__ kout,
+                                                        uint32_t *__restrict__ vout, int64_t n,
+                                                        int shift,
+                                                        const uint32_t *__restrict__ hist,
+                                                        int ntiles)
+{
+	__shared__ uint32_t cnt[kSortWaves][256];
+	for (int w = 0; w < kSortWaves; w++)
+		cnt[w][threadIdx.x] = 0;
+	__syncthreads();
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+	const int64_t base = (int64_t)blockIdx.x * kTile;
+	uint32_t kk[kSortItems], vv[kSortItems];
+	uint16_t loc[kSortItems];
+	for (int it = 0; it < kSortItems; it++) {
+		const int64_t e = base + tile_pos(wave, it, lane);
+		const bool valid = e < n;
+		kk[it] = valid ? kin[e] : 0u;
+		vv[it] = valid ? vin[e] : 0u;
+		const uint32_t d = (kk[it] >> shift) & 0xFFu;
+		uint64_t peers = __ballot(valid);
+#pragma unroll
+		for (int b = 0; b < 8; b++) {
+			const uint64_t bal = __ballot((d >> b) & 1u);
+			peers &= ((d >> b) & 1u) ? bal : ~bal;
+		}
+		if (valid) {
+			const uint32_t c = cnt[wave][d];
+			loc[it] = (uint16_t)(c + (uint32_t)__popcll(peers & lt_mask));
+			if ((peers & lt_mask) == 0)  // lowest lane of the peer group
+				cnt[wave][d] = c + (uint32_t)__popcll(peers);
+		}
+	}
+	__syncthreads();
+	{
+		const int d = threadIdx.x;
+		uint32_t run = hist[(int64_t)d * ntiles + blockIdx.x];
+		for (int w = 0; w < kSortWaves; w++) {
+			const uint32_t t = cnt[w][d];
+			cnt[w][d] = run;
+			run += t;
+		}
+	}
+	__syncthreads();
+	for (int it = 0; it < kSortItems; it++) {
+		const int64_t e = base + tile_pos(wave, it, lane);
+		if (e < n) {
+			const uint32_t d = (kk[it] >> shift) & 0xFFu;
+			const uint32_t pos = cnt[wave][d] + loc[it];
+			kout[pos] = kk[it];
+			vout[pos] = vv[it];
+		}
+	}
+}
+
+// first element of each digit after the final pass (for material counts)
+__global__ void k_digit_starts(const uint32_t *__restrict__ hist, int ntiles, uint32_t *out)
+{
+	out[threadIdx.x] = hist[(int64_t)threadIdx.x * ntiles];
+}
+
+// Lookups in sorted order: position i holds lookup id ids[i]; (E, mat) are
+// re-derived from the id's LCG state.
+template <int GRID, int PREC, bool DETAIL>
+__global__ __launch_bounds__(256) void k_lookup_ids(Tables T, int64_t n,
+                                                    const uint32_t *__restrict__ ids,
+                                                    int64_t first, uint64_t seed0, DetailOut det,
+                                                    unsigned long long *vsum,
+                                                    unsigned long long *nfb)
+{
+	const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	uint32_t v = 0, fb = 0;
+	if (i < n) {
+		const uint32_t id = ids[i];
+		uint64_t s = ff_lcg(seed0, 2ULL * (uint64_t)(first + id));
+		const double p = lcg_double(s);
+		const int mat = pick_mat(s, T.thr);
+		const int64_t idx = lookup_index<GRID>(T, p);
+		double macro[5];
+		int am;
+		if (PREC == XSB_XS_FP64) {
+			macro_xs_fp64<GRID>(T, p, mat, idx, macro);
+			am = argmax5(macro);
+		} else {
+			am = macro_xs_mixed<GRID, PREC>(T, p, mat, idx, macro, &fb);
+		}
+		v = (uint32_t)am + 1u;
+		if (DETAIL) {
+			const int64_t o = id;
+			if (det.energy) det.energy[o] = p;
+			if (det.mat) det.mat[o] = mat;
+			if (det.index) det.index[o] = idx;
+			if (det.macro)
+				for (int c = 0; c < 5; c++) det.macro[5 * o + c] = macro[c];
+			if (det.argmax) det.argmax[o] = am;
+			if (det.flag) det.flag[o] = (int32_t)fb;
+		}
+	}
+	block_add_u64<256>(v, vsum);
+	if (PREC != XSB_XS_FP64 && nfb)
+		block_add_u64<256>(fb, nfb);
+}
+
+template <int PREC>
+static void launch_ids_prec(Device *d, hipStream_t s, int64_t n, const uint32_t *ids, int64_t first,
+                            uint64_t seed, const DetailOut *det, unsigned long long *vsum,
+                            unsigned long long *nfb)
+{
+	const unsigned nb = (unsigned)((n + 255) / 256);
+	DetailOut dz{};
+	const DetailOut &dd = det ? *det : dz;
+#define XSB_LAUNCH(G)                                                                          \
+	do {                                                                                       \
+		if (det)                                                                               \
+			hipLaunchKernelGGL((k_lookup_ids<G, PREC, true>), dim3(nb), dim3(256), 0, s, d->T, \
+			                   n, ids, first, seed, dd, vsum, nfb);                            \
+		else                                                                                   \
+			hipLaunchKernelGGL((k_lookup_ids<G, PREC, false>), dim3(nb), dim3(256), 0, s,      \
+			                   d->T, n, ids, first, seed, dd, vsum, nfb);                      \
+	} while (0)
+	switch (d->T.grid_type) {
+	case kUnionized: XSB_LAUNCH(kUnionized); break;
+	case kNuclide: XSB_LAUNCH(kNuclide); break;
+	default: XSB_LAUNCH(kHash); break;
+	}
+#undef XSB_LAUNCH
+}
+
+static int scan_hist(uint32_t *hist, int64_t len, uint32_t *part, hipStream_t s)
+{
+	const int nparts = (int)((len + kScanChunk - 1) / kScanChunk);
+	if (nparts > 256 * kScanItems)
+		return fail(XSB_ERR_INVALID, "histogram too large to scan (%lld entries)", (long long)len);
+	hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(256), 0, s, hist, len, part);
+	hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(256), 0, s, part, nparts);
+	hipLaunchKernelGGL(k_scan_apply, dim3(nparts), dim3(256), 0, s, hist, len, part);
+	XSB_HIP(hipGetLastError());
+	return XSB_OK;
+}
+
+int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, xsb_result *r,
+                     unsigned long long *vsum, unsigned long long *nfb)
+{
+	const int64_t n = a->lookups;
+	hipStream_t s = d->stream;
+	const int k = r->kernel_id;
+	const int mode = (k == XSB_K_MAT_E_SORT) ? kModeMatE
+	                 : (k == XSB_K_FUEL_LAUNCH || k == XSB_K_FUEL_SORT) ? kModeFuel
+	                                                                    : kModeMat;
+	const int passes = (mode == kModeMatE) ? 3 : 1;
+	if (n == 0) {
+		XSB_HIP(hipEventRecord(d->ev[1], s));
+		XSB_HIP(hipEventRecord(d->ev[2], s));
+		return XSB_OK;
+	}
+	const int ntiles = (int)((n + kTile - 1) / kTile);
+	const int64_t hlen = (int64_t)256 * ntiles;
+	const int nparts = (int)((hlen + kScanChunk - 1) / kScanChunk);
+	const size_t bytes = (size_t)n * 16 + (size_t)hlen * 4 + (size_t)nparts * 4 + 2048 + 1024;
+	void *buf = nullptr;
+	int rc = scratch_reserve(d, 0, bytes, &buf);
+	if (rc)
+		return rc;
+	uint32_t *kA = (uint32_t *)buf, *vA = kA + n, *kB = vA + n, *vB = kB + n;
+	uint32_t *hist = vB + n, *part = hist + hlen, *starts = part + nparts + 16;
+
+	hipLaunchKernelGGL(k_keys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, a->first_lookup, n,
+	                   a->seed, mode, kA, vA, hist, ntiles);
+	XSB_HIP(hipGetLastError());
+	for (int pass = 0; pass < passes; pass++) {
+		const int shift = 8 * pass;
+		if (pass > 0) {
+			hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kSortBlock), 0, s, kA, n, shift, hist,
+			                   ntiles);
+			XSB_HIP(hipGetLastError());
+		}
+		if ((rc = scan_hist(hist, hlen, part, s)))
+			return rc;
+		hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(kSortBlock), 0, s, kA, vA, kB, vB, n,
+		                   shift, hist, ntiles);
+		XSB_HIP(hipGetLastError());
+		uint32_t *t = kA;
+		kA = kB;
+		kB = t;
+		t = vA;
+		vA = vB;
+		vB = t;
+	}
+	hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(256), 0, s, hist, ntiles, starts);
+	XSB_HIP(hipGetLastError());
+	XSB_HIP(hipEventRecord(d->ev[1], s));
+	if (a->xs_precision == XSB_XS_FP64)
+		launch_ids_prec<XSB_XS_FP64>(d, s, n, vA, a->first_lookup, a->seed, det, vsum, nfb);
+	else if (a->xs_precision == XSB_XS_FP32)
+		launch_ids_prec<XSB_XS_FP32>(d, s, n, vA, a->first_lookup, a->seed, det, vsum, nfb);
+	else
+		launch_ids_prec<XSB_XS_BF16>(d, s, n, vA, a->first_lookup, a->seed, det, vsum, nfb);
+	XSB_HIP(hipGetLastError());
+	XSB_HIP(hipEventRecord(d->ev[2], s));
+
+	// material counts from the last pass's digit starts
+	uint32_t st[256];
+	XSB_HIP(hipMemcpyAsync(st, starts, sizeof(st), hipMemcpyDeviceToHost, s));
+	XSB_HIP(hipStreamSynchronize(s));
+	if (mode != kModeFuel) {
+		for (int dgt = 0; dgt < 256; dgt++) {
+			const uint32_t next = (dgt == 255) ? (uint32_t)n : st[dgt + 1];
+			const int m = (mode == kModeMat) ? dgt : (dgt >> (kEnergyBits - 16));
+			if (m < kMaterials)
+				r->material_counts[m] += (int64_t)(next - st[dgt]);
+		}
+	} else {
+		r->material_counts[0] = (int64_t)st[1];
+	}
+	return XSB_OK;
+}
+
+}  // namespace xsb
