@@ -88,6 +88,11 @@ typedef struct {
 	int64_t *out_index; /* unionized-grid index, or hash bin */
 	double *out_macro;
 	int32_t *out_argmax;
+	/* mixed precision only: the per-lookup error bound B with
+	 * |macro_c - FP64 macro_c| <= B for every channel, and 1 where the
+	 * lookup was re-evaluated in FP64 */
+	double *out_bound;
+	int32_t *out_fallback;
 } xsb_event_args;
 
 /* History-based run (openmp-threading/Simulation.c:116-238) */

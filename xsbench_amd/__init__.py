@@ -102,7 +102,8 @@ class XsbEventArgs(C.Structure):
         ("lookups", C.c_int64), ("first_lookup", C.c_int64), ("seed", C.c_uint64),
         ("kernel_id", C.c_int32), ("xs_precision", C.c_int32),
         ("out_energy", C.c_void_p), ("out_mat", C.c_void_p), ("out_index", C.c_void_p),
-        ("out_macro", C.c_void_p), ("out_argmax", C.c_void_p),
+        ("out_macro", C.c_void_p), ("out_argmax", C.c_void_p), ("out_bound", C.c_void_p),
+        ("out_fallback", C.c_void_p),
     ]
 
 
@@ -354,7 +355,10 @@ class Device:
         if detail:
             out = {"energy": np.empty(lookups, np.float64), "mat": np.empty(lookups, np.int32),
                    "index": np.empty(lookups, np.int64), "macro": np.empty((lookups, 5), np.float64),
-                   "argmax": np.empty(lookups, np.int32)}
+                   "argmax": np.empty(lookups, np.int32), "bound": np.empty(lookups, np.float64),
+                   "fallback": np.empty(lookups, np.int32)}
+            a.out_bound = out["bound"].ctypes.data
+            a.out_fallback = out["fallback"].ctypes.data
             a.out_energy = out["energy"].ctypes.data
             a.out_mat = out["mat"].ctypes.data
             a.out_index = out["index"].ctypes.data

@@ -439,10 +439,11 @@ struct DetailBufs {
 
 static int detail_alloc(Device *d, int64_t n, const xsb_event_args *a, DetailBufs *b)
 {
-	b->any = a->out_energy || a->out_mat || a->out_index || a->out_macro || a->out_argmax;
+	b->any = a->out_energy || a->out_mat || a->out_index || a->out_macro || a->out_argmax ||
+	         a->out_bound || a->out_fallback;
 	if (!b->any)
 		return XSB_OK;
-	const size_t per = 8 + 4 + 8 + 40 + 4 + 4;
+	const size_t per = 8 + 4 + 8 + 40 + 4 + 4 + 8 + 4;
 	void *base = nullptr;
 	int rc = scratch_reserve(d, 3, (size_t)n * per + 4096, &base);
 	if (rc)
@@ -458,7 +459,9 @@ static int detail_alloc(Device *d, int64_t n, const xsb_event_args *a, DetailBuf
 	p += (size_t)n * 4;
 	b->dev.argmax = a->out_argmax ? (int32_t *)p : nullptr;
 	p += (size_t)n * 4;
-	b->dev.flag = nullptr;
+	b->dev.bound = a->out_bound ? (double *)p : nullptr;
+	p += (size_t)n * 8;
+	b->dev.flag = a->out_fallback ? (int32_t *)p : nullptr;
 	return XSB_OK;
 }
 
@@ -477,6 +480,16 @@ static int detail_copy_back(Device *d, int64_t n, const xsb_event_args *a, const
 		XSB_HIP(hipMemcpyAsync(a->out_mat, b->dev.mat, n * 4, hipMemcpyDeviceToHost, s));
 	if (a->out_argmax)
 		XSB_HIP(hipMemcpyAsync(a->out_argmax, b->dev.argmax, n * 4, hipMemcpyDeviceToHost, s));
+	if (a->out_bound) {
+		if (a->xs_precision == XSB_XS_FP64)
+			XSB_HIP(hipMemsetAsync(b->dev.bound, 0, n * 8, s));
+		XSB_HIP(hipMemcpyAsync(a->out_bound, b->dev.bound, n * 8, hipMemcpyDeviceToHost, s));
+	}
+	if (a->out_fallback) {
+		if (a->xs_precision == XSB_XS_FP64)
+			XSB_HIP(hipMemsetAsync(b->dev.flag, 0, n * 4, s));
+		XSB_HIP(hipMemcpyAsync(a->out_fallback, b->dev.flag, n * 4, hipMemcpyDeviceToHost, s));
+	}
 	XSB_HIP(hipStreamSynchronize(s));
 	return XSB_OK;
 }

@@ -66,6 +66,7 @@ struct DetailOut {
 	double *macro;
 	int32_t *argmax;
 	int32_t *flag;  // mixed precision: 1 where the lookup was re-evaluated in FP64
+	double *bound;  // mixed precision: the error bound B of each lookup
 };
 
 // ---- launchers (xsb_lookup.hip) ----

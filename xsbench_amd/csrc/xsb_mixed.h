@@ -84,7 +84,8 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float t)
 // HISTORY mode needs for its > 1.0 test).  *fell_back is set to 1 on fallback.
 template <int GRID, int PREC, bool HISTORY = false>
 __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat, int64_t idx,
-                                              double macro[5], uint32_t *fell_back)
+                                              double macro[5], uint32_t *fell_back,
+                                              double *bound = nullptr)
 {
 	constexpr int REC = (PREC == XSB_XS_FP32) ? kRec32Doubles : kRec16Doubles;
 	constexpr float kU = (PREC == XSB_XS_FP32) ? 0x1p-24f : 0x1p-8f;
@@ -112,7 +113,7 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 		const float c = ceff[j];
 #pragma unroll
 		for (int k = 0; k < 5; k++) {
-			const float x = xh[k] - fr * (xh[k] - xl[k]);
+			const float x = __builtin_fmaf(-fr, xh[k] - xl[k], xh[k]);
 			kahan_add(sum[k], comp[k], x * c);
 		}
 		A += cbnd[j] * (1.0f + 2.0f * fabsf(fr));
@@ -124,6 +125,8 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 		s[k] = (double)sum[k];
 	const int am = argmax5(s);
 	const double B = 1.01 * (double)A * ((double)kU + 12.0 * 0x1p-24);
+	if (bound)
+		*bound = B;
 	bool ambiguous = false;
 #pragma unroll
 	for (int k = 0; k < 5; k++) {

@@ -259,12 +259,13 @@ __global__ __launch_bounds__(256) void k_lookup_ids(Tables T, int64_t n,
 		const int mat = pick_mat(s, T.thr);
 		const int64_t idx = lookup_index<GRID>(T, p);
 		double macro[5];
+		double bnd = 0.0;
 		int am;
 		if (PREC == XSB_XS_FP64) {
 			macro_xs_fp64<GRID>(T, p, mat, idx, macro);
 			am = argmax5(macro);
 		} else {
-			am = macro_xs_mixed<GRID, PREC>(T, p, mat, idx, macro, &fb);
+			am = macro_xs_mixed<GRID, PREC>(T, p, mat, idx, macro, &fb, DETAIL ? &bnd : nullptr);
 		}
 		v = (uint32_t)am + 1u;
 		if (DETAIL) {
@@ -276,6 +277,7 @@ __global__ __launch_bounds__(256) void k_lookup_ids(Tables T, int64_t n,
 				for (int c = 0; c < 5; c++) det.macro[5 * o + c] = macro[c];
 			if (det.argmax) det.argmax[o] = am;
 			if (det.flag) det.flag[o] = (int32_t)fb;
+			if (det.bound) det.bound[o] = bnd;
 		}
 	}
 	block_add_u64<256>(v, vsum);
