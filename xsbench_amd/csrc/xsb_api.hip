@@ -2,6 +2,8 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -57,6 +59,30 @@ static void pick_thresholds(double thr[kMaterials])
 	}
 }
 
+// {A, C} of the LCG skip by d * 256^k for k < 8, d < 256, computed by the
+// reference's square-and-multiply (openmp-threading/Simulation.c:472-499)
+static void ff_table(uint64_t *tab)
+{
+	const uint64_t A = 2806196910506780709ULL;
+	for (int k = 0; k < 8; k++) {
+		for (int dgt = 0; dgt < 256; dgt++) {
+			uint64_t n = (uint64_t)dgt << (8 * k);
+			uint64_t a = A, c = 1, an = 1, cn = 0;
+			while (n) {
+				if (n & 1) {
+					an *= a;
+					cn = cn * a + c;
+				}
+				c *= (a + 1);
+				a *= a;
+				n >>= 1;
+			}
+			tab[2 * (256 * k + dgt)] = an;
+			tab[2 * (256 * k + dgt) + 1] = cn;
+		}
+	}
+}
+
 template <typename T>
 static int upload_array(T **dst, const T *src, int64_t count, int64_t *bytes_acc)
 {
@@ -84,7 +110,8 @@ static void release(Device *d)
 	if (d->stream)
 		(void)hipStreamSynchronize(d->stream);
 	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_grid,  d->d_egrid,    d->d_xs32, d->d_xs16,
-	                d->d_xs16_scale, d->d_num_nucs, d->d_mats, d->d_concs, d->d_concs32, d->d_acc};
+	                d->d_xs16_scale, d->d_num_nucs, d->d_mats, d->d_concs, d->d_concs32, d->d_acc,
+	                d->d_fftab};
 	for (void *p : ptrs)
 		if (p)
 			(void)hipFree(p);
@@ -297,6 +324,39 @@ static int build_compact(Device *d, const xsb_tables *t, uint32_t flags)
 	return XSB_OK;
 }
 
+// rows of n ints -> rows of `pitch` ints (padding already zeroed)
+__global__ void k_repitch(const int32_t *__restrict__ src, int64_t rows, int32_t n, int32_t pitch,
+                          int32_t *__restrict__ dst)
+{
+	const int64_t total = rows * n;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const int64_t r = i / n;
+		dst[r * pitch + (i - r * n)] = src[i];
+	}
+}
+
+// Copies a rows x n int grid into the device's pitched layout through a
+// bounded staging buffer (contiguous host-to-device copies, re-pitched on
+// the device).
+static int upload_pitched(Device *d, const int32_t *host, int64_t rows, int64_t n, int64_t pitch)
+{
+	const int64_t stage_rows = std::max<int64_t>(1, ((int64_t)64 << 20) / (n * 4));
+	void *stage = nullptr;
+	int rc = scratch_reserve(d, 0, (size_t)std::min(rows, stage_rows) * n * 4, &stage);
+	if (rc)
+		return rc;
+	for (int64_t r0 = 0; r0 < rows; r0 += stage_rows) {
+		const int64_t nr = std::min(stage_rows, rows - r0);
+		XSB_HIP(hipMemcpy(stage, host + r0 * n, (size_t)nr * n * 4, hipMemcpyHostToDevice));
+		hipLaunchKernelGGL(k_repitch, dim3(2048), dim3(256), 0, nullptr, (const int32_t *)stage, nr,
+		                   (int32_t)n, (int32_t)pitch, d->d_index + r0 * pitch);
+		XSB_HIP(hipGetLastError());
+		XSB_HIP(hipDeviceSynchronize());
+	}
+	return XSB_OK;
+}
+
 static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 {
 	int rc = validate_tables(t);
@@ -335,9 +395,19 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 		                       &d->bytes_union)))
 			return rc;
 	}
+	const int64_t pitch = (n_iso + 3) & ~(int64_t)3;  // 16-byte aligned rows
 	if (t->grid_type != XSB_NUCLIDE) {
-		if ((rc = upload_array(&d->d_index, t->index_grid, t->length_index_grid, &d->bytes_index)))
+		const int64_t rows = t->length_index_grid / n_iso;
+		const size_t bytes = (size_t)rows * pitch * sizeof(int32_t);
+		if (hipMalloc((void **)&d->d_index, bytes) != hipSuccess) {
+			(void)hipGetLastError();
+			d->d_index = nullptr;
+			return fail(XSB_ERR_OOM, "hipMalloc of %zu bytes (index grid) failed", bytes);
+		}
+		XSB_HIP(hipMemset(d->d_index, 0, bytes));
+		if ((rc = upload_pitched(d, t->index_grid, rows, n_iso, pitch)))
 			return rc;
+		d->bytes_index = (int64_t)bytes;
 	}
 	if (hipMalloc((void **)&d->d_acc, 64 * sizeof(unsigned long long)) != hipSuccess) {
 		(void)hipGetLastError();
@@ -351,7 +421,7 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	if (d->d_index) {
 		const int32_t hi = (t->grid_type == XSB_HASH) ? (int32_t)(n_gp - 2) : (int32_t)(n_gp - 1);
 		hipLaunchKernelGGL(k_index_range, dim3(4096), dim3(256), 0, d->stream, d->d_index,
-		                   (int64_t)t->length_index_grid, hi, d->d_acc + 60);
+		                   (int64_t)(t->length_index_grid / n_iso) * pitch, hi, d->d_acc + 60);
 		XSB_HIP(hipGetLastError());
 	}
 	if (d->d_ueg) {
@@ -378,11 +448,43 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	T.concs = d->d_concs;
 	T.n_union = (t->grid_type == XSB_UNIONIZED) ? t->length_unionized_energy_array : 0;
 	T.n_iso = (int32_t)n_iso;
+	T.idx_pitch = (int32_t)pitch;
+	T.variant = kVariantDefault;
+	if (const char *v = std::getenv("XSB_LOOKUP_VARIANT"))
+		T.variant = (int32_t)std::strtol(v, nullptr, 0);
+	// per material: the longest run of consecutive ids that starts at a
+	// multiple of 4, cut to a multiple of 4 (read as 16-byte vectors)
+	for (int m = 0; m < kMaterials; m++) {
+		const int32_t *ml = t->mats + m * t->max_num_nucs;
+		const int nn = t->num_nucs[m];
+		int best_s = 0, best_l = 0;
+		for (int j = 0; j < nn; j++) {
+			if (ml[j] % 4 != 0)
+				continue;
+			int l = 1;
+			while (j + l < nn && ml[j + l] == ml[j] + l)
+				l++;
+			l &= ~3;
+			if (l > best_l) {
+				best_l = l;
+				best_s = j;
+			}
+		}
+		T.vec_start[m] = best_s;
+		T.vec_len[m] = best_l;
+	}
 	T.n_gp = (int32_t)n_gp;
 	T.hash_bins = t->hash_bins;
 	T.max_num_nucs = t->max_num_nucs;
 	T.grid_type = t->grid_type;
 	T.hash_du = (t->hash_bins > 0) ? 1.0 / t->hash_bins : 0.0;
+	{
+		std::vector<uint64_t> tab(8 * 256 * 2);
+		ff_table(tab.data());
+		if ((rc = upload_array(&d->d_fftab, tab.data(), (int64_t)tab.size(), nullptr)))
+			return rc;
+		T.ff_tab = d->d_fftab;
+	}
 	pick_thresholds(T.thr);
 	for (int i = 1; i < kMaterials; i++)
 		if (!(T.thr[i] > T.thr[i - 1]))

@@ -22,10 +22,10 @@ namespace xsb {
 
 constexpr int kBatchBlock = 256;
 
-__device__ __forceinline__ void batch_sample(uint64_t seed, const double *thr, int64_t i,
-                                             double &p, int &mat)
+__device__ __forceinline__ void batch_sample(const Tables &T, uint64_t seed, const double *thr,
+                                             int64_t i, double &p, int &mat)
 {
-	uint64_t s = ff_lcg(seed, 2ULL * (uint64_t)i);
+	uint64_t s = ff_jump(seed, 2ULL * (uint64_t)i, T.ff_tab);
 	p = lcg_double(s);
 	const double roll = lcg_double(s);
 	int m = 0;
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_one(Tables T, int64_t siz
 	if (i < size) {
 		double p;
 		int mat;
-		batch_sample(seed, thr, i, p, mat);
+		batch_sample(T, seed, thr, i, p, mat);
 		v = (uint32_t)lookup_one<GRID, PREC>(T, p, mat, &fb) + 1u;
 	}
 	block_add_u64<kBatchBlock>(v, vsum);
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(Tables T, Batc
 		for (int j = threadIdx.x; j < n; j += kBatchBlock) {
 			double p;
 			int mat;
-			batch_sample(seed, thr, t0 + j, p, mat);
+			batch_sample(T, seed, thr, t0 + j, p, mat);
 			s_mat[j] = (uint8_t)mat;
 			atomicAdd(&s_cnt[mat + 1], 1u);
 		}
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(Tables T, Batc
 		for (int j = threadIdx.x; j < n; j += kBatchBlock) {
 			double p;
 			int mat;
-			batch_sample(seed, thr, t0 + s_slot[j], p, mat);
+			batch_sample(T, seed, thr, t0 + s_slot[j], p, mat);
 			uint32_t f1 = 0;
 			v += (uint32_t)lookup_one<GRID, PREC>(T, p, mat, &f1) + 1u;
 			fb += f1;

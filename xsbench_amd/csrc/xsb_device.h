@@ -17,6 +17,12 @@ constexpr int kPointDoubles = 6;  // energy + 5 XS channels (NuclideGridPoint)
 
 enum GridType : int { kUnionized = 0, kNuclide = 1, kHash = 2 };
 
+// Lookup-path variants (all give identical results; chosen per table set).
+constexpr int kVariantWaveSearch = 1;   // bracket the wave's energies, short per-lane search
+constexpr int kVariantVecIndex = 2;     // 16-byte index loads over consecutive-id runs
+constexpr int kVariantChunk16 = 4;      // index words fetched 16 (not 32) at a time
+constexpr int kVariantDefault = kVariantVecIndex | kVariantChunk16;
+
 // Device view of the tables.  Passed by value as a kernel argument.
 struct Tables {
 	const double *__restrict__ ueg;         // unionized energies [n_union]
@@ -30,15 +36,22 @@ struct Tables {
 	const int32_t *__restrict__ mats;       // [12][max_num_nucs]
 	const double *__restrict__ concs;       // [12][max_num_nucs]
 	const float *__restrict__ concs32;      // FP32 copy of concs
+	const uint64_t *__restrict__ ff_tab;    // skip-ahead table: 8 digits x 256 x {A, C}
 	int64_t n_union;
 	int32_t n_iso;
+	int32_t idx_pitch;  // row pitch of the device index grid (ints, multiple of 4)
 	int32_t n_gp;
 	int32_t hash_bins;
 	int32_t max_num_nucs;
 	int32_t grid_type;
-	int32_t pad0;
+	int32_t variant;  // kVariant* bits: lookup-path choices (tuning, same results)
 	double thr[kMaterials];  // pick_mat thresholds, reverse-order partial sums
 	double hash_du;          // 1.0 / hash_bins, as computed by the reference
+	// per material: positions [vec_start, vec_start + vec_len) of its list
+	// hold consecutive nuclide ids starting at a multiple of 4, vec_len a
+	// multiple of 4 (the large fuel's ids 68..) -- read as 16-byte vectors
+	int32_t vec_start[kMaterials];
+	int32_t vec_len[kMaterials];
 };
 
 // ---------------------------------------------------------------------------
@@ -69,6 +82,24 @@ __device__ __forceinline__ uint64_t ff_lcg(uint64_t seed, uint64_t n)
 		n >>= 1;
 	}
 	return (an * seed + cn) & kLcgMask;
+}
+
+// fast_forward_LCG through a radix-256 table: tab[2 (256 k + d)] = {A, C}
+// of the skip by d * 256^k.  Skips of one LCG commute, so applying the
+// digits of n in any order gives the reference's FF(seed, n) exactly.
+__device__ __forceinline__ uint64_t ff_jump(uint64_t seed, uint64_t n,
+                                            const uint64_t *__restrict__ tab)
+{
+	n &= kLcgMask;
+	uint64_t s = seed;
+	for (int k = 0; n; k++, n >>= 8) {
+		const uint32_t d = (uint32_t)(n & 0xFFu);
+		if (d) {
+			const ulonglong2 ac = *reinterpret_cast<const ulonglong2 *>(tab + 2 * (256 * k + d));
+			s = ac.x * s + ac.y;
+		}
+	}
+	return s & kLcgMask;
 }
 
 // pick_mat: the reference returns the first i with roll < thr[i], where
@@ -102,6 +133,50 @@ __device__ __forceinline__ int32_t grid_search(int32_t n, double q, const double
 	return lo;
 }
 
+// The reference's bisection restricted to [lo, hi]: returns the same index
+// as grid_search whenever that index is known to lie in [lo, hi - 1].
+__device__ __forceinline__ int32_t grid_search_range(int32_t lo, int32_t hi, double q,
+                                                     const double *__restrict__ A)
+{
+	while (hi - lo > 1) {
+		int32_t mid = lo + ((hi - lo) >> 1);
+		if (A[mid] > q)
+			hi = mid;
+		else
+			lo = mid;
+	}
+	return lo;
+}
+
+__device__ __forceinline__ double wave_min_f64(double x)
+{
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1)
+		x = fmin(x, __shfl_xor(x, off, 64));
+	return x;
+}
+
+__device__ __forceinline__ double wave_max_f64(double x)
+{
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1)
+		x = fmax(x, __shfl_xor(x, off, 64));
+	return x;
+}
+
+// grid_search for a full wave (all 64 lanes active): the result is monotone
+// in q, so the brackets of the wave's smallest and largest energy (found
+// once, on wave-uniform values) enclose every lane's result, and each lane
+// bisects only between them.
+__device__ __forceinline__ int32_t grid_search_wave(int32_t n, double q,
+                                                    const double *__restrict__ A)
+{
+	const double qmin = wave_min_f64(q), qmax = wave_max_f64(q);
+	const int32_t L = grid_search_range(0, n - 1, qmin, A);
+	const int32_t H = grid_search_range(L, n - 1, qmax, A);
+	return grid_search_range(L, H + 1, q, A);
+}
+
 // bisection on energies stored with a stride (in doubles)
 template <int STRIDE>
 __device__ __forceinline__ int32_t grid_search_nuclide(double q, const double *__restrict__ E,
@@ -130,14 +205,14 @@ __device__ __forceinline__ int32_t nuclide_lower(const Tables &T, const double *
 	const int32_t n_gp = T.n_gp;
 	int32_t lower;
 	if (GRID == kUnionized) {
-		lower = T.index_grid[idx * T.n_iso + nuc];
+		lower = T.index_grid[idx * T.idx_pitch + nuc];
 	} else if (GRID == kNuclide) {
 		lower = grid_search_nuclide<STRIDE>(p, E_row, 0, n_gp - 1);
 	} else {
-		const int32_t u_low = T.index_grid[idx * T.n_iso + nuc];
+		const int32_t u_low = T.index_grid[idx * T.idx_pitch + nuc];
 		const int32_t u_high = (idx == T.hash_bins - 1)
 		                           ? n_gp - 1
-		                           : T.index_grid[(idx + 1) * T.n_iso + nuc] + 1;
+		                           : T.index_grid[(idx + 1) * T.idx_pitch + nuc] + 1;
 		const double e_low = E_row[(int64_t)u_low * STRIDE];
 		const double e_high = E_row[(int64_t)u_high * STRIDE];
 		if (p <= e_low)
@@ -206,6 +281,129 @@ __device__ __forceinline__ void macro_xs_fp64(const Tables &T, double p, int mat
 	}
 }
 
+// Points k0, k0+1, k0+2 of one nuclide are shared by the wave (scalar loads);
+// each lane interpolates between points sel and sel+1 (sel = 0 or 1).
+__device__ __forceinline__ void accumulate_fp64_select(const double *__restrict__ P, int sel,
+                                                       double p, double conc, double macro[5])
+{
+	double a[3 * kPointDoubles];
+#pragma unroll
+	for (int w = 0; w < 3 * kPointDoubles; w++)
+		a[w] = P[w];
+	double lo[kPointDoubles], hi[kPointDoubles];
+#pragma unroll
+	for (int w = 0; w < kPointDoubles; w++) {
+		lo[w] = sel ? a[kPointDoubles + w] : a[w];
+		hi[w] = sel ? a[2 * kPointDoubles + w] : a[kPointDoubles + w];
+	}
+	const double fr = (hi[0] - p) / (hi[0] - lo[0]);
+#pragma unroll
+	for (int c = 0; c < 5; c++)
+		macro[c] += (hi[1 + c] - fr * (hi[1 + c] - lo[1 + c])) * conc;
+}
+
+// One nuclide of the wave-uniform unionized path: bracket index k per lane;
+// the points are read through the scalar cache when the lanes' brackets
+// coincide or differ by one point, else per lane.
+__device__ __forceinline__ void ueg_wave_nuclide(const Tables &T, int nuc, int k, double p,
+                                                 double conc, double macro[5])
+{
+	const int32_t n_gp = T.n_gp;
+	if (k == n_gp - 1)
+		k--;
+	const int k0 = __builtin_amdgcn_readfirstlane(k);
+	const int d = k - k0;
+	const double *P = T.grid + ((int64_t)nuc * n_gp + k0) * kPointDoubles;
+	if (__ballot(d != 0) == 0ULL)
+		accumulate_fp64(P, p, conc, macro);
+	else if (__ballot(d != 0 && d != 1) == 0ULL)
+		accumulate_fp64_select(P, d, p, conc, macro);
+	else if (__ballot(d != 0 && d != -1) == 0ULL)
+		accumulate_fp64_select(P - kPointDoubles, d + 1, p, conc, macro);
+	else
+		accumulate_fp64(T.grid + ((int64_t)nuc * n_gp + k) * kPointDoubles, p, conc, macro);
+}
+
+// calculate_macro_xs, unionized grid, FP64, for a wave whose lanes all hold
+// material m (the material-sorted schedules).  The material list, the
+// concentrations and the shared grid points are wave-uniform (scalar cache).
+// Each lane's index-grid words are fetched CHUNK at a time, all in flight
+// together, so a lane's row is read once per chunk instead of once per
+// nuclide; with kVariantVecIndex the material's run of consecutive ids is
+// fetched as 16-byte vectors.  Arithmetic and summation order are those of
+// macro_xs_fp64.
+template <int CHUNK>
+__device__ __forceinline__ void ueg_wave_scattered(const Tables &T, const int32_t *__restrict__ ml,
+                                                   const double *__restrict__ cl,
+                                                   const int32_t *__restrict__ row, int j0,
+                                                   int j1, double p, double macro[5])
+{
+	for (; j0 + CHUNK <= j1; j0 += CHUNK) {
+		int kk[CHUNK];
+#pragma unroll
+		for (int u = 0; u < CHUNK; u++)
+			kk[u] = row[ml[j0 + u]];
+#pragma unroll
+		for (int u = 0; u < CHUNK; u++)
+			ueg_wave_nuclide(T, ml[j0 + u], kk[u], p, cl[j0 + u], macro);
+	}
+	for (; j0 < j1; j0++)
+		ueg_wave_nuclide(T, ml[j0], row[ml[j0]], p, cl[j0], macro);
+}
+
+template <int CHUNK>
+__device__ __forceinline__ void macro_ueg_wave_c(const Tables &T, double p, int m, int64_t idx,
+                                                 double macro[5])
+{
+#pragma unroll
+	for (int c = 0; c < 5; c++)
+		macro[c] = 0.0;
+	const int nn = T.num_nucs[m];
+	const int32_t *__restrict__ ml = T.mats + m * T.max_num_nucs;
+	const double *__restrict__ cl = T.concs + m * T.max_num_nucs;
+	const int32_t *__restrict__ row = T.index_grid + idx * T.idx_pitch;
+	if (!(T.variant & kVariantVecIndex)) {
+		ueg_wave_scattered<CHUNK>(T, ml, cl, row, 0, nn, p, macro);
+		return;
+	}
+	const int vs = T.vec_start[m], ve = vs + T.vec_len[m];
+	ueg_wave_scattered<CHUNK>(T, ml, cl, row, 0, vs, p, macro);
+	int j0 = vs;
+	for (; j0 + CHUNK <= ve; j0 += CHUNK) {
+		const int4 *q = reinterpret_cast<const int4 *>(row + ml[j0]);
+		int4 kk[CHUNK / 4];
+#pragma unroll
+		for (int u = 0; u < CHUNK / 4; u++)
+			kk[u] = q[u];
+#pragma unroll
+		for (int u = 0; u < CHUNK / 4; u++) {
+			const int nuc = ml[j0 + 4 * u];
+			ueg_wave_nuclide(T, nuc, kk[u].x, p, cl[j0 + 4 * u], macro);
+			ueg_wave_nuclide(T, nuc + 1, kk[u].y, p, cl[j0 + 4 * u + 1], macro);
+			ueg_wave_nuclide(T, nuc + 2, kk[u].z, p, cl[j0 + 4 * u + 2], macro);
+			ueg_wave_nuclide(T, nuc + 3, kk[u].w, p, cl[j0 + 4 * u + 3], macro);
+		}
+	}
+	for (; j0 < ve; j0 += 4) {
+		const int4 k4 = *reinterpret_cast<const int4 *>(row + ml[j0]);
+		const int nuc = ml[j0];
+		ueg_wave_nuclide(T, nuc, k4.x, p, cl[j0], macro);
+		ueg_wave_nuclide(T, nuc + 1, k4.y, p, cl[j0 + 1], macro);
+		ueg_wave_nuclide(T, nuc + 2, k4.z, p, cl[j0 + 2], macro);
+		ueg_wave_nuclide(T, nuc + 3, k4.w, p, cl[j0 + 3], macro);
+	}
+	ueg_wave_scattered<CHUNK>(T, ml, cl, row, ve, nn, p, macro);
+}
+
+__device__ __forceinline__ void macro_ueg_wave(const Tables &T, double p, int m, int64_t idx,
+                                               double macro[5])
+{
+	if (T.variant & kVariantChunk16)
+		macro_ueg_wave_c<16>(T, p, m, idx, macro);
+	else
+		macro_ueg_wave_c<32>(T, p, m, idx, macro);
+}
+
 // argmax with max = -1.0 and strict '>' (Simulation.c:100-110)
 __device__ __forceinline__ int argmax5(const double m[5])
 {
@@ -219,6 +417,20 @@ __device__ __forceinline__ int argmax5(const double m[5])
 		}
 	}
 	return am;
+}
+
+// Block order for kernels whose neighbouring blocks share data: the
+// dispatcher sends block b to XCD b % 8, so consecutive logical blocks are
+// regrouped into runs of `run` blocks per XCD, and each XCD's L2 sees a
+// contiguous stretch of the (sorted) work.  Bijective on [0, gridDim.x).
+__device__ __forceinline__ unsigned xcd_block(unsigned run)
+{
+	const unsigned b = blockIdx.x, G = gridDim.x, per = 8u * run;
+	const unsigned R = (G / per) * per;
+	if (b >= R)
+		return b;
+	const unsigned xcd = b & 7u, i = b >> 3;
+	return (i / run) * per + xcd * run + (i % run);
 }
 
 // ---------------------------------------------------------------------------

@@ -49,6 +49,7 @@ struct Device {
 	double *d_concs = nullptr;
 	float *d_concs32 = nullptr;
 	unsigned long long *d_acc = nullptr;  // 64 accumulators
+	uint64_t *d_fftab = nullptr;          // LCG skip-ahead table
 	int64_t bytes_fp64 = 0, bytes_fp32 = 0, bytes_bf16 = 0, bytes_index = 0, bytes_union = 0;
 	int32_t num_nucs_host[kMaterials] = {0};
 	Scratch scratch[4];

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kBlock) void k_event_fused(Tables T, int64_t first,
 	const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
 	uint32_t v = 0;
 	if (i < n) {
-		uint64_t s = ff_lcg(seed0, 2ULL * (uint64_t)(first + i));
+		uint64_t s = ff_jump(seed0, 2ULL * (uint64_t)(first + i), T.ff_tab);
 		const double p = lcg_double(s);
 		const int mat = pick_mat(s, T.thr);
 		const int64_t idx = lookup_index<GRID>(T, p);
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void k_sample(Tables T, int64_t first, int6
 	const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
 	if (i >= n)
 		return;
-	uint64_t s = ff_lcg(seed0, 2ULL * (uint64_t)(first + i));
+	uint64_t s = ff_jump(seed0, 2ULL * (uint64_t)(first + i), T.ff_tab);
 	E[i] = lcg_double(s);
 	mat[i] = pick_mat(s, T.thr);
 }
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_history(Tables T, int64_t particles,
 	const int64_t pid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
 	uint32_t v = 0, fb = 0;
 	if (pid < particles) {
-		uint64_t s = ff_lcg(seed0, history_offset(pid, lookups));
+		uint64_t s = ff_jump(seed0, history_offset(pid, lookups), T.ff_tab);
 		double p = lcg_double(s);
 		int mat = pick_mat(s, T.thr);
 		for (int it = 0; it < lookups; it++) {
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_balg(Tables T, int64_t first, int64_
 	const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
 	uint32_t b = 0;
 	if (i < n) {
-		uint64_t s = ff_lcg(seed0, 2ULL * (uint64_t)(first + i));
+		uint64_t s = ff_jump(seed0, 2ULL * (uint64_t)(first + i), T.ff_tab);
 		const double p = lcg_double(s);
 		const int mat = pick_mat(s, T.thr);
 		const int nn = T.num_nucs[mat];
@@ -337,13 +337,13 @@ __global__ __launch_bounds__(kBlock) void k_balg(Tables T, int64_t first, int64_
 				int seen = (lower > 0) + (lower + 1 < n_gp - 1);
 				b += (uint32_t)(pr * f.e + 2 * (f.e + f.xs) - seen * f.e);
 			} else {
-				const int32_t u_low = T.index_grid[idx * T.n_iso + nuc];
+				const int32_t u_low = T.index_grid[idx * T.idx_pitch + nuc];
 				int32_t u_high;
 				b += (uint32_t)f.idx;
 				if (idx == T.hash_bins - 1) {
 					u_high = n_gp - 1;
 				} else {
-					u_high = T.index_grid[(idx + 1) * T.n_iso + nuc] + 1;
+					u_high = T.index_grid[(idx + 1) * T.idx_pitch + nuc] + 1;
 					b += (uint32_t)f.idx;
 				}
 				const double e_low = Er[(int64_t)u_low * kPointDoubles];

@@ -9,7 +9,8 @@
 //
 //   k = 2, 4 : key = material                       (material segments)
 //   k = 3, 5 : key = (material != fuel)             (fuel first)
-//   k = 6    : key = material << 20 | energy bits   (material, then energy)
+//   k = 6    : key = material << 16 | energy bits   (material, then energy)
+// Digits are 10 bits wide, so the material-energy key sorts in two passes.
 #include "xsb_internal.h"
 #include "xsb_mixed.h"
 
@@ -19,9 +20,13 @@ constexpr int kSortBlock = 256;
 constexpr int kSortWaves = kSortBlock / 64;
 constexpr int kSortItems = 16;
 constexpr int kTile = kSortBlock * kSortItems;  // 4096 elements per tile
-constexpr int kEnergyBits = 20;
+constexpr int kEnergyBits = 16;
+constexpr int kDigitBits = 10;
+constexpr int kBins = 1 << kDigitBits;
+constexpr uint32_t kDigitMask = kBins - 1;
 constexpr int kScanItems = 16;
 constexpr int kScanChunk = 256 * kScanItems;  // 4096 entries per scan block
+constexpr unsigned kXcdRun = 16;              // lookup blocks per XCD run
 
 enum SortMode : int { kModeMat = 0, kModeFuel = 1, kModeMatE = 2 };
 
@@ -33,7 +38,7 @@ __device__ __forceinline__ int tile_pos(int wave, int item, int lane)
 __device__ __forceinline__ uint32_t sample_key(const Tables &T, uint64_t seed0, int64_t lookup,
                                                int mode)
 {
-	uint64_t s = ff_lcg(seed0, 2ULL * (uint64_t)lookup);
+	uint64_t s = ff_jump(seed0, 2ULL * (uint64_t)lookup, T.ff_tab);
 	const double p = lcg_double(s);
 	const int mat = pick_mat(s, T.thr);
 	if (mode == kModeFuel)
@@ -51,8 +56,9 @@ __global__ __launch_bounds__(kSortBlock) void k_keys(Tables T, int64_t first, in
                                                      uint64_t seed0, int mode, uint32_t *keys,
                                                      uint32_t *vals, uint32_t *hist, int ntiles)
 {
-	__shared__ uint32_t h[256];
-	h[threadIdx.x] = 0;
+	__shared__ uint32_t h[kBins];
+	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
+		h[d] = 0;
 	__syncthreads();
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int64_t base = (int64_t)blockIdx.x * kTile;
@@ -62,28 +68,31 @@ __global__ __launch_bounds__(kSortBlock) void k_keys(Tables T, int64_t first, in
 			const uint32_t k = sample_key(T, seed0, first + e, mode);
 			keys[e] = k;
 			vals[e] = (uint32_t)e;
-			atomicAdd(&h[k & 0xFFu], 1u);
+			atomicAdd(&h[k & kDigitMask], 1u);
 		}
 	}
 	__syncthreads();
-	hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
+		hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
 }
 
 __global__ __launch_bounds__(kSortBlock) void k_hist(const uint32_t *__restrict__ keys, int64_t n,
                                                      int shift, uint32_t *hist, int ntiles)
 {
-	__shared__ uint32_t h[256];
-	h[threadIdx.x] = 0;
+	__shared__ uint32_t h[kBins];
+	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
+		h[d] = 0;
 	__syncthreads();
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int64_t base = (int64_t)blockIdx.x * kTile;
 	for (int it = 0; it < kSortItems; it++) {
 		const int64_t e = base + tile_pos(wave, it, lane);
 		if (e < n)
-			atomicAdd(&h[(keys[e] >> shift) & 0xFFu], 1u);
+			atomicAdd(&h[(keys[e] >> shift) & kDigitMask], 1u);
 	}
 	__syncthreads();
-	hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
+		hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
 }
 
 // ---- exclusive scan of the digit-major histogram (reduce, scan, apply) ----
@@ -135,22 +144,27 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t *__restrict_
 		part[blockIdx.x] = tot;
 }
 
-// single block: exclusive scan of up to 256 * kScanItems partials
+// single block: exclusive scan of the partials, kScanChunk at a time
 __global__ __launch_bounds__(256) void k_scan_parts(uint32_t *part, int nparts)
 {
-	uint32_t v[kScanItems];
-	uint32_t s = 0;
-	for (int it = 0; it < kScanItems; it++) {
-		const int i = threadIdx.x * kScanItems + it;
-		v[it] = i < nparts ? part[i] : 0u;
-		s += v[it];
-	}
-	uint32_t off = block_excl_scan(s, nullptr);
-	for (int it = 0; it < kScanItems; it++) {
-		const int i = threadIdx.x * kScanItems + it;
-		if (i < nparts)
-			part[i] = off;
-		off += v[it];
+	uint32_t carry = 0;
+	for (int base = 0; base < nparts; base += kScanChunk) {
+		uint32_t v[kScanItems];
+		uint32_t s = 0;
+		for (int it = 0; it < kScanItems; it++) {
+			const int i = base + threadIdx.x * kScanItems + it;
+			v[it] = i < nparts ? part[i] : 0u;
+			s += v[it];
+		}
+		uint32_t tot;
+		uint32_t off = block_excl_scan(s, &tot) + carry;
+		for (int it = 0; it < kScanItems; it++) {
+			const int i = base + threadIdx.x * kScanItems + it;
+			if (i < nparts)
+				part[i] = off;
+			off += v[it];
+		}
+		carry += tot;
 	}
 }
 
@@ -174,7 +188,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *a, int64_t n,
 	}
 }
 
-// Stable scatter of one 8-bit digit.  Tile order is (wave, item, lane); each
+// Stable scatter of one digit.  Tile order is (wave, item, lane); each
 // wave ranks its items in that order with ballot-built peer masks and
 // per-wave digit counters in LDS, then waves are offset per digit.
 __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t *__restrict__ kin,
@@ -185,9 +199,10 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t *__restri
                                                         const uint32_t *__restrict__ hist,
                                                         int ntiles)
 {
-	__shared__ uint32_t cnt[kSortWaves][256];
+	__shared__ uint32_t cnt[kSortWaves][kBins];
 	for (int w = 0; w < kSortWaves; w++)
-		cnt[w][threadIdx.x] = 0;
+		for (int d = threadIdx.x; d < kBins; d += kSortBlock)
+			cnt[w][d] = 0;
 	__syncthreads();
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
@@ -199,10 +214,10 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t *__restri
 		const bool valid = e < n;
 		kk[it] = valid ? kin[e] : 0u;
 		vv[it] = valid ? vin[e] : 0u;
-		const uint32_t d = (kk[it] >> shift) & 0xFFu;
+		const uint32_t d = (kk[it] >> shift) & kDigitMask;
 		uint64_t peers = __ballot(valid);
 #pragma unroll
-		for (int b = 0; b < 8; b++) {
+		for (int b = 0; b < kDigitBits; b++) {
 			const uint64_t bal = __ballot((d >> b) & 1u);
 			peers &= ((d >> b) & 1u) ? bal : ~bal;
 		}
@@ -214,8 +229,7 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t *__restri
 		}
 	}
 	__syncthreads();
-	{
-		const int d = threadIdx.x;
+	for (int d = threadIdx.x; d < kBins; d += kSortBlock) {
 		uint32_t run = hist[(int64_t)d * ntiles + blockIdx.x];
 		for (int w = 0; w < kSortWaves; w++) {
 			const uint32_t t = cnt[w][d];
@@ -227,7 +241,7 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t *__restri
 	for (int it = 0; it < kSortItems; it++) {
 		const int64_t e = base + tile_pos(wave, it, lane);
 		if (e < n) {
-			const uint32_t d = (kk[it] >> shift) & 0xFFu;
+			const uint32_t d = (kk[it] >> shift) & kDigitMask;
 			const uint32_t pos = cnt[wave][d] + loc[it];
 			kout[pos] = kk[it];
 			vout[pos] = vv[it];
@@ -238,11 +252,14 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t *__restri
 // first element of each digit after the final pass (for material counts)
 __global__ void k_digit_starts(const uint32_t *__restrict__ hist, int ntiles, uint32_t *out)
 {
-	out[threadIdx.x] = hist[(int64_t)threadIdx.x * ntiles];
+	for (int d = threadIdx.x; d < kBins; d += blockDim.x)
+		out[d] = hist[(int64_t)d * ntiles];
 }
 
 // Lookups in sorted order: position i holds lookup id ids[i]; (E, mat) are
-// re-derived from the id's LCG state.
+// re-derived from the id's LCG state.  Lanes past n repeat lookup n-1 (in the
+// same wave) and contribute nothing, so every lane stays active for the
+// wave-wide search and the wave-uniform material path.
 template <int GRID, int PREC, bool DETAIL>
 __global__ __launch_bounds__(256) void k_lookup_ids(Tables T, int64_t n,
                                                     const uint32_t *__restrict__ ids,
@@ -250,35 +267,49 @@ __global__ __launch_bounds__(256) void k_lookup_ids(Tables T, int64_t n,
                                                     unsigned long long *vsum,
                                                     unsigned long long *nfb)
 {
-	const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	uint32_t v = 0, fb = 0;
-	if (i < n) {
-		const uint32_t id = ids[i];
-		uint64_t s = ff_lcg(seed0, 2ULL * (uint64_t)(first + id));
-		const double p = lcg_double(s);
-		const int mat = pick_mat(s, T.thr);
-		const int64_t idx = lookup_index<GRID>(T, p);
-		double macro[5];
-		double bnd = 0.0;
-		int am;
-		if (PREC == XSB_XS_FP64) {
-			macro_xs_fp64<GRID>(T, p, mat, idx, macro);
-			am = argmax5(macro);
+	const int64_t i = (int64_t)xcd_block(kXcdRun) * 256 + threadIdx.x;
+	const bool valid = i < n;
+	const uint32_t id = ids[valid ? i : n - 1];
+	uint64_t s = ff_jump(seed0, 2ULL * (uint64_t)(first + id), T.ff_tab);
+	const double p = lcg_double(s);
+	const int mat = pick_mat(s, T.thr);
+	double macro[5];
+	double bnd = 0.0;
+	uint32_t fb = 0;
+	int64_t idx;
+	int am;
+	if (PREC == XSB_XS_FP64 && GRID == kUnionized) {
+		const int m0 = __builtin_amdgcn_readfirstlane(mat);
+		if (__ballot(mat != m0) == 0ULL) {
+			idx = (T.variant & kVariantWaveSearch) ? grid_search_wave((int32_t)T.n_union, p, T.ueg)
+			                                       : lookup_index<GRID>(T, p);
+			macro_ueg_wave(T, p, m0, idx, macro);
 		} else {
-			am = macro_xs_mixed<GRID, PREC>(T, p, mat, idx, macro, &fb, DETAIL ? &bnd : nullptr);
+			idx = lookup_index<GRID>(T, p);
+			macro_xs_fp64<GRID>(T, p, mat, idx, macro);
 		}
-		v = (uint32_t)am + 1u;
-		if (DETAIL) {
-			const int64_t o = id;
-			if (det.energy) det.energy[o] = p;
-			if (det.mat) det.mat[o] = mat;
-			if (det.index) det.index[o] = idx;
-			if (det.macro)
-				for (int c = 0; c < 5; c++) det.macro[5 * o + c] = macro[c];
-			if (det.argmax) det.argmax[o] = am;
-			if (det.flag) det.flag[o] = (int32_t)fb;
-			if (det.bound) det.bound[o] = bnd;
-		}
+		am = argmax5(macro);
+	} else if (PREC == XSB_XS_FP64) {
+		idx = lookup_index<GRID>(T, p);
+		macro_xs_fp64<GRID>(T, p, mat, idx, macro);
+		am = argmax5(macro);
+	} else {
+		idx = lookup_index<GRID>(T, p);
+		am = macro_xs_mixed<GRID, PREC>(T, p, mat, idx, macro, &fb, DETAIL ? &bnd : nullptr);
+	}
+	if (!valid)
+		fb = 0;
+	const uint32_t v = valid ? (uint32_t)am + 1u : 0u;
+	if (DETAIL && valid) {
+		const int64_t o = id;
+		if (det.energy) det.energy[o] = p;
+		if (det.mat) det.mat[o] = mat;
+		if (det.index) det.index[o] = idx;
+		if (det.macro)
+			for (int c = 0; c < 5; c++) det.macro[5 * o + c] = macro[c];
+		if (det.argmax) det.argmax[o] = am;
+		if (det.flag) det.flag[o] = (int32_t)fb;
+		if (det.bound) det.bound[o] = bnd;
 	}
 	block_add_u64<256>(v, vsum);
 	if (PREC != XSB_XS_FP64 && nfb)
@@ -313,8 +344,6 @@ static void launch_ids_prec(Device *d, hipStream_t s, int64_t n, const uint32_t 
 static int scan_hist(uint32_t *hist, int64_t len, uint32_t *part, hipStream_t s)
 {
 	const int nparts = (int)((len + kScanChunk - 1) / kScanChunk);
-	if (nparts > 256 * kScanItems)
-		return fail(XSB_ERR_INVALID, "histogram too large to scan (%lld entries)", (long long)len);
 	hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(256), 0, s, hist, len, part);
 	hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(256), 0, s, part, nparts);
 	hipLaunchKernelGGL(k_scan_apply, dim3(nparts), dim3(256), 0, s, hist, len, part);
@@ -331,16 +360,16 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	const int mode = (k == XSB_K_MAT_E_SORT) ? kModeMatE
 	                 : (k == XSB_K_FUEL_LAUNCH || k == XSB_K_FUEL_SORT) ? kModeFuel
 	                                                                    : kModeMat;
-	const int passes = (mode == kModeMatE) ? 3 : 1;
+	const int passes = (mode == kModeMatE) ? 2 : 1;
 	if (n == 0) {
 		XSB_HIP(hipEventRecord(d->ev[1], s));
 		XSB_HIP(hipEventRecord(d->ev[2], s));
 		return XSB_OK;
 	}
 	const int ntiles = (int)((n + kTile - 1) / kTile);
-	const int64_t hlen = (int64_t)256 * ntiles;
+	const int64_t hlen = (int64_t)kBins * ntiles;
 	const int nparts = (int)((hlen + kScanChunk - 1) / kScanChunk);
-	const size_t bytes = (size_t)n * 16 + (size_t)hlen * 4 + (size_t)nparts * 4 + 2048 + 1024;
+	const size_t bytes = (size_t)n * 16 + (size_t)hlen * 4 + (size_t)nparts * 4 + 64 + 4 * kBins;
 	void *buf = nullptr;
 	int rc = scratch_reserve(d, 0, bytes, &buf);
 	if (rc)
@@ -352,7 +381,7 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	                   a->seed, mode, kA, vA, hist, ntiles);
 	XSB_HIP(hipGetLastError());
 	for (int pass = 0; pass < passes; pass++) {
-		const int shift = 8 * pass;
+		const int shift = kDigitBits * pass;
 		if (pass > 0) {
 			hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kSortBlock), 0, s, kA, n, shift, hist,
 			                   ntiles);
@@ -383,13 +412,14 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	XSB_HIP(hipEventRecord(d->ev[2], s));
 
 	// material counts from the last pass's digit starts
-	uint32_t st[256];
+	static_assert(kEnergyBits + 4 <= 2 * kDigitBits, "material-energy key fits two digits");
+	uint32_t st[kBins];
 	XSB_HIP(hipMemcpyAsync(st, starts, sizeof(st), hipMemcpyDeviceToHost, s));
 	XSB_HIP(hipStreamSynchronize(s));
 	if (mode != kModeFuel) {
-		for (int dgt = 0; dgt < 256; dgt++) {
-			const uint32_t next = (dgt == 255) ? (uint32_t)n : st[dgt + 1];
-			const int m = (mode == kModeMat) ? dgt : (dgt >> (kEnergyBits - 16));
+		for (int dgt = 0; dgt < kBins; dgt++) {
+			const uint32_t next = (dgt == kBins - 1) ? (uint32_t)n : st[dgt + 1];
+			const int m = (mode == kModeMat) ? dgt : (dgt >> (kEnergyBits - kDigitBits));
 			if (m < kMaterials)
 				r->material_counts[m] += (int64_t)(next - st[dgt]);
 		}
