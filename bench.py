@@ -76,6 +76,99 @@ def cpu_baseline(inp, sd):
                       f"{dt:.2f} s"}
 
 
+def batched_variants(X, dev):
+    """The batched workload: 512 independent batches, sizes log-uniform in
+    [1e3, 2e6], each with its own stream and material CDF, on the same large
+    unionized tables; sorted pipeline (hipGraph) with the autotuned plan.
+    Per-batch sums must agree between precisions."""
+    import numpy as np
+    sizes, seeds, thr = X.make_batches(2024, 512, 1000, 2_000_000)
+    total = int(sizes.sum())
+    out = {"batched_batches": len(sizes), "batched_lookups": total}
+    ref_vs = None
+    for name, prec, window in (("fp64", X.XS_FP64, 0), ("fp32", X.XS_FP32, 0),
+                               ("fp32_window16M", X.XS_FP32, 16_000_000)):
+        plan = {}
+        dev.batched(sizes, seeds, thr, precision=prec, mode=0, window=window, plan=plan)  # tunes
+        best = None
+        for _ in range(3):
+            r, vs, done = dev.batched(sizes, seeds, thr, precision=prec, mode=0, window=window)
+            if best is None or r["ms_total"] < best[0]["ms_total"]:
+                best = (r, vs, done)
+        r, vs, done = best
+        if ref_vs is None:
+            ref_vs = vs
+        elif not np.array_equal(vs, ref_vs):
+            raise SystemExit(f"batched {name}: per-batch sums differ from FP64")
+        ms = r["ms_total"]
+        out[f"batched_{name}_lookups_per_s"] = round(total / (ms * 1e-3), 1)
+        out[f"batched_{name}_ms"] = round(ms, 3)
+        out[f"batched_{name}_sort_ms"] = round(r["ms_sample"], 3)
+        out[f"batched_{name}_latency_p50_ms"] = round(float(np.percentile(done, 50)), 3)
+        out[f"batched_{name}_latency_p99_ms"] = round(float(np.percentile(done, 99)), 3)
+        out[f"batched_{name}_plan"] = {k: plan[k] for k in ("tile", "stage", "sort", "mapping",
+                                                            "windows")}
+        log(f"batched {name}: {total:,} lookups in {ms:.3f} ms = {total / ms / 1e6:.3f} G lookups/s "
+            f"(sort {r['ms_sample']:.3f} ms), batch completion p50 "
+            f"{out[f'batched_{name}_latency_p50_ms']} ms p99 {out[f'batched_{name}_latency_p99_ms']} ms, "
+            f"plan {out[f'batched_{name}_plan']}, tuning {plan['ms_tuning']:.0f} ms")
+    return out
+
+
+def hash_variants(X):
+    """Large problem on the hash grid (10,000 bins), 17M lookups: FP64, FP32
+    and BF16 tables, material-energy sorted schedule; identical hashes, and
+    the relative error of each channel of the macro XS against the FP64
+    path (bit-identical to the reference) on the first 1M lookups."""
+    import numpy as np
+    inp = X.make_inputs("large", "hash", "event")
+    sd = X.grid_init(inp)
+    out = {}
+    with X.Device(inp, sd, build_fp32=True, build_bf16=True) as dev:
+        ref = None
+        for name, prec in (("fp64", X.XS_FP64), ("fp32", X.XS_FP32), ("bf16", X.XS_BF16)):
+            dev.event(LOOKUPS, kernel_id=6, precision=prec)
+            rr = [dev.event(LOOKUPS, kernel_id=6, precision=prec) for _ in range(3)]
+            if any(r["hash"] != HASH_LARGE for r in rr):
+                raise SystemExit(f"hash grid {name}: hash {rr[0]['hash']} != {HASH_LARGE}")
+            ms = min(r["ms_total"] for r in rr)
+            out[f"hash_{name}_k6_lookups_per_s"] = round(LOOKUPS / (ms * 1e-3), 1)
+            out[f"hash_{name}_k6_fp64_reevaluations"] = rr[0]["n_fallback"]
+            _, det = dev.event(1_000_000, kernel_id=6, precision=prec, detail=True)
+            if ref is None:
+                ref = det["macro"]
+            else:
+                rel = np.abs(det["macro"] - ref) / np.maximum(np.abs(ref), 1e-300)
+                out[f"hash_{name}_rel_err_max"] = [float(f"{v:.3e}") for v in rel.max(axis=0)]
+                out[f"hash_{name}_rel_err_mean"] = [float(f"{v:.3e}") for v in rel.mean(axis=0)]
+            log(f"hash grid {name} k=6: {ms:.3f} ms, {rr[0]['n_fallback']} FP64 re-evaluations")
+    return out
+
+
+def history_variants(X):
+    """History mode, XL grid with 45,000 gridpoints, hash grid, 1M particles x
+    34 lookups: one particle per lane (k=0) against material-energy sorted
+    rounds (k=6), FP64 and FP32 tables; identical verification sums."""
+    inp = X.make_inputs("XL", "hash", "history", gridpoints=45000, particles=1_000_000)
+    sd = X.grid_init(inp)
+    out = {}
+    with X.Device(inp, sd, build_fp32=True) as dev:
+        want = None
+        for name, k, prec in (("fp64_k0", 0, X.XS_FP64), ("fp64_k6", 6, X.XS_FP64),
+                              ("fp32_k6", 6, X.XS_FP32)):
+            dev.history(1_000_000, 34, kernel_id=k, precision=prec)
+            rr = [dev.history(1_000_000, 34, kernel_id=k, precision=prec) for _ in range(2)]
+            if want is None:
+                want = rr[0]["vsum"]
+            if any(r["vsum"] != want for r in rr):
+                raise SystemExit(f"history {name}: sum {rr[0]['vsum']} != {want}")
+            ms = min(r["ms_total"] for r in rr)
+            out[f"history_xl45k_{name}_lookups_per_s"] = round(34e6 / (ms * 1e-3), 1)
+            log(f"history XL-45000 hash {name}: {ms:.3f} ms per 34M lookups")
+        out["history_xl45k_hash"] = want % 999983
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +244,9 @@ def main():
             variants[f"{name}_k6_lookups_per_s"] = round(LOOKUPS / (ms * 1e-3), 1)
             variants[f"{name}_k6_fp64_reevaluations"] = rr[0]["n_fallback"]
             log(f"{name} k=6: {ms:.3f} ms, {rr[0]['n_fallback']} FP64 re-evaluations")
+        variants.update(batched_variants(X, dev))
+        variants.update(hash_variants(X))
+        variants.update(history_variants(X))
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -99,7 +99,10 @@ typedef struct {
 typedef struct {
 	int64_t particles;
 	int32_t lookups_per_particle;
-	int32_t kernel_id;    /* 0: one particle per lane; 6: rounds, material-sorted */
+	int32_t kernel_id;    /* 0: one particle per lane, its lookups in sequence;
+	                         6: rounds -- lookup r of every particle, ordered by material
+	                         and energy, one round after another (a replayed hipGraph);
+	                         -1: rounds from 65536 particles on, else 0 */
 	int32_t xs_precision; /* XSB_XS_* */
 	int32_t reserved0;
 	uint64_t seed;                 /* STARTING_SEED */
@@ -111,17 +114,51 @@ typedef struct {
  * material CDF (no reference counterpart; the producer of many small event
  * batches).  Batch b holds sizes[b] lookups; lookup i of batch b samples
  * s = FF(seeds[b], 2i), E = LCG(s), roll = LCG(s), mat = first m with
- * roll < thresholds[12 b + m]. */
+ * roll < thresholds[12 b + m].
+ *
+ * mode 0 runs one pipeline captured once in a hipGraph and replayed while
+ * the call shape (batch count, total lookups, plan, precision) is unchanged:
+ * a key kernel, an LSD radix sort of all lookups by (sort window, material,
+ * energy), then one persistent lookup kernel whose workgroups take tiles of
+ * the sorted order from per-XCD work queues (stealing across XCDs when
+ * their own queue runs dry).  Batches are grouped in order into sort windows
+ * of about `window` lookups; windows run in order, so a smaller window trades
+ * throughput for earlier completion of the first batches.  Zero fields of
+ * the plan (tile, stage, sort, mapping) are chosen by an autotuner keyed on the
+ * nuclide count, grid type, precision and batch-size histogram, whose choice
+ * is cached on the handle. */
+typedef struct {
+	int32_t tile;      /* lookups per work-queue item */
+	int32_t stage;     /* index-grid words per lane staged in LDS per chunk (8 or 16) */
+	int32_t sort;      /* 1: material, 2: material then energy */
+	int32_t windows;   /* sort windows the batches were grouped into */
+	int32_t tuned;     /* 1 if the autotuner ran during this call */
+	int32_t graph;     /* 1 if the pipeline ran as a hipGraph */
+	int32_t mapping;   /* 1: one lane per lookup; 2: wave-uniform path where a wave's
+	                      lanes share the material (unionized FP64) */
+	int32_t reserved0;
+	double ms_tuning;  /* host wall time spent tuning during this call */
+} xsb_batch_plan;
+
 typedef struct {
 	int32_t n_batches;
 	int32_t xs_precision;
 	const int64_t *sizes;      /* host [n_batches] */
 	const uint64_t *seeds;     /* host [n_batches] */
-	const double *thresholds;  /* host [n_batches x 12], thresholds[12b+11] >= 1 */
-	int32_t mode;              /* 0 persistent kernel + graph, 1 one launch per batch */
-	int32_t tile;              /* lookups per work item (0 = autotuned) */
+	const double *thresholds;  /* host [n_batches x 12], non-decreasing, thresholds[12b+11] >= 1 */
+	int32_t mode;              /* 0 sorted pipeline (above), 1 one launch per batch in lookup
+	                              order, 2 persistent queue in lookup order (each tile ordered
+	                              by material in LDS) */
+	int32_t tile;              /* lookups per work item (0 = autotuned in mode 0, 4096 in mode 2) */
+	int32_t stage;             /* mode 0: 8 or 16 (0 = autotuned) */
+	int32_t sort;              /* mode 0: 1 or 2 (0 = autotuned) */
+	int64_t window;            /* mode 0: lookups per sort window (0 = one window) */
+	int32_t eager;             /* mode 0: 1 = launch the kernels directly instead of the graph */
+	int32_t mapping;           /* mode 0: 1 lane per lookup, 2 wave-uniform (0 = autotuned) */
 	uint64_t *out_vsum;        /* optional host [n_batches] */
-	double *out_done_ms;       /* optional host [n_batches]: completion time of each batch */
+	double *out_done_ms;       /* optional host [n_batches]: completion time of each batch,
+	                              from the start of the call's first kernel */
+	xsb_batch_plan *out_plan;  /* optional: the plan that ran */
 } xsb_batched_args;
 
 typedef struct {

@@ -123,6 +123,21 @@ void release_device_memory(SimulationData SD);
  * returns the process exit code: 1 when the hash is not the published one. */
 int xsbench_main(int argc, char *argv[]);
 
+/* ---- batched lookups (no reference counterpart): many independent
+ * batches, each with its own sample stream and material CDF ---- */
+/* The batched workload's generator: sizes log-uniform in [lo, hi], one
+ * stream seed and 12 cumulative material thresholds per batch. */
+void xsbench_make_batches(uint64_t seed, int n_batches, long lo, long hi, long *sizes,
+                          uint64_t *batch_seeds, double *thresholds);
+/* Lookup i of a batch (energy, material) from its stream. */
+void xsbench_batch_sample(uint64_t batch_seed, const double *thresholds, long i, double *energy,
+                          int *mat);
+/* Runs every batch on the GPU; returns the un-modded sum over all batches. */
+unsigned long long run_batched_simulation(Inputs in, SimulationData SD, int n_batches,
+                                          const long *sizes, const uint64_t *batch_seeds,
+                                          const double *thresholds, unsigned long long *out_vsum,
+                                          double *out_done_ms);
+
 /* ---- options of this implementation (no reference counterpart) ---- */
 typedef struct {
 	int xs_precision;  /* 0 FP64 (bit-exact), 1 FP32 tables, 2 BF16 tables */

@@ -124,28 +124,45 @@ def test_small_one_million_lookups_golden(golden_hashes):
 # history mode
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("name", ["small_unionized", "small_hash", "large_hash", "xl45k_hash"])
-def test_history_particles_match_reference(name):
+@pytest.mark.parametrize("kernel", [0, 6])
+def test_history_particles_match_reference(name, kernel):
     inp, sd, dev = data(name)
     g = golden_npz(f"history_{name}.npz")
     n = len(g["sums"])
-    r, out = dev.history(n, int(g["lookups"]), detail=True)
+    r, out = dev.history(n, int(g["lookups"]), kernel_id=kernel, detail=True)
     assert np.array_equal(out["sums"], g["sums"])
     assert np.array_equal(out["final_seed"], g["final_seed"])
 
 
+@pytest.mark.parametrize("kernel", [0, 6])
 @pytest.mark.parametrize("name,size", [("small_unionized", "small"), ("large_unionized", "large")])
-def test_history_published_hashes(name, size):
+def test_history_published_hashes(name, size, kernel):
     inp, sd, dev = data(name)
-    r = dev.history(500_000, 34)
-    print(f"[gpu-test] history {name}: {r['ms_total']:.2f} ms", flush=True)
+    r = dev.history(500_000, 34, kernel_id=kernel)
+    print(f"[gpu-test] history {name} k={kernel}: {r['ms_total']:.2f} ms", flush=True)
     assert r["hash"] == X.PUBLISHED[("history", size)]
 
 
-def test_history_xl45k_golden(golden_hashes):
+@pytest.mark.parametrize("kernel", [0, 6])
+def test_history_xl45k_golden(golden_hashes, kernel):
     inp, sd, dev = data("xl45k_hash")
     for key in ("history_xl45k_hash_20k", "history_xl45k_hash_1M"):
         h = golden_hashes[key]
-        assert dev.history(h["particles"], h["lookups"])["vsum"] == h["vsum"]
+        r = dev.history(h["particles"], h["lookups"], kernel_id=kernel)
+        print(f"[gpu-test] {key} k={kernel}: {r['ms_total']:.2f} ms", flush=True)
+        assert r["vsum"] == h["vsum"]
+
+
+def test_history_rounds_graph_replays():
+    """the captured chain of rounds replays with the same result, with other
+    work between replays, and recaptures for a new particle count"""
+    inp, sd, dev = data("small_hash")
+    want = dev.history(70000, 34, kernel_id=0)["vsum"]
+    for rep in range(3):
+        assert dev.history(70000, 34, kernel_id=6)["vsum"] == want, rep
+        dev.event(20000, kernel_id=X.K_MAT_E_SORT)
+    assert dev.history(1000, 5, kernel_id=6)["vsum"] == dev.history(1000, 5, kernel_id=0)["vsum"]
+    assert dev.history(1, 34, kernel_id=6)["vsum"] == dev.history(1, 34, kernel_id=0)["vsum"]
 
 
 # ---------------------------------------------------------------------------
@@ -183,10 +200,11 @@ def test_mixed_precision_error_per_channel(name, prec, p9999):
 
 
 @pytest.mark.parametrize("prec", [X.XS_FP32, X.XS_BF16])
-def test_mixed_precision_history_identical(prec):
+@pytest.mark.parametrize("kernel", [0, 6])
+def test_mixed_precision_history_identical(prec, kernel):
     inp, sd, dev = data("large_hash")
     r64 = dev.history(20000, 34)
-    r = dev.history(20000, 34, precision=prec)
+    r = dev.history(20000, 34, precision=prec, kernel_id=kernel)
     assert r["vsum"] == r64["vsum"]
 
 
@@ -239,17 +257,73 @@ def test_lookup_on_device_buffers(orc):
     assert int(vs[0]) == int(want_am.sum()) + n
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+def _batch_truth(orc, oi, osd, sizes, seeds, thr):
+    return [orc.run_batch(oi, osd, int(seeds[b]), thr[b], int(sizes[b])) for b in range(len(sizes))]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("prec", [X.XS_FP64, X.XS_FP32])
 def test_batched_per_batch_hashes(orc, mode, prec):
     inp, sd, dev = data("small_unionized")
     oi, osd = as_oracle(inp, sd)
     sizes, seeds, thr = orc.batch_plan(2024, 24, 1000, 60000)
-    r, vs, done = dev.batched(sizes, seeds, thr, precision=prec, mode=mode)
-    for b in range(len(sizes)):
-        assert int(vs[b]) == orc.run_batch(oi, osd, int(seeds[b]), thr[b], int(sizes[b]))
+    plan = {}
+    r, vs, done = dev.batched(sizes, seeds, thr, precision=prec, mode=mode, plan=plan)
+    assert [int(v) for v in vs] == _batch_truth(orc, oi, osd, sizes, seeds, thr)
     assert r["vsum"] == int(vs.sum())
     assert np.all(done > 0)
+    if mode == 0:
+        assert plan["sort"] in (1, 2) and plan["stage"] in (8, 16) and plan["tile"] >= 256
+        assert plan["graph"] == 1
+
+
+@pytest.mark.parametrize("name", ["small_unionized", "small_hash", "small_nuclide"])
+@pytest.mark.parametrize("tile,stage,sort,window", [(256, 8, 1, 0), (2048, 16, 2, 0),
+                                                    (8192, 8, 2, 25000), (512, 16, 1, 1)])
+def test_batched_sorted_plans_identical(orc, name, tile, stage, sort, window):
+    """every plan and sort window of the sorted pipeline gives the oracle's per-batch sums"""
+    inp, sd, dev = data(name)
+    oi, osd = as_oracle(inp, sd)
+    sizes, seeds, thr = orc.batch_plan(77, 12, 100, 30000)
+    sizes[3] = 0  # an empty batch
+    plan = {}
+    r, vs, done = dev.batched(sizes, seeds, thr, mode=0, tile=tile, stage=stage, sort=sort,
+                              window=window, plan=plan)
+    assert [int(v) for v in vs] == _batch_truth(orc, oi, osd, sizes, seeds, thr)
+    assert plan["tile"] == tile and plan["sort"] == sort and plan["tuned"] == 0
+    assert done[3] == 0 and np.all(np.delete(done, 3) > 0)
+    if window:
+        assert plan["windows"] > 1
+
+
+def test_batched_graph_replays_match_eager(orc):
+    """the captured pipeline replayed several times, with other work and
+    synchronisations between replays, gives the eager result every time"""
+    inp, sd, dev = data("small_unionized")
+    oi, osd = as_oracle(inp, sd)
+    sizes, seeds, thr = orc.batch_plan(5, 16, 1000, 40000)
+    _, v_eager, _ = dev.batched(sizes, seeds, thr, mode=0, tile=1024, stage=16, sort=2, eager=True)
+    assert [int(v) for v in v_eager] == _batch_truth(orc, oi, osd, sizes, seeds, thr)
+    for rep in range(4):
+        plan = {}
+        _, vs, _ = dev.batched(sizes, seeds, thr, mode=0, tile=1024, stage=16, sort=2, plan=plan)
+        assert plan["graph"] == 1
+        assert np.array_equal(vs, v_eager), rep
+        assert dev.event(30000, kernel_id=X.K_MAT_E_SORT)["hash"] > 0  # eager work in between
+    # new batch contents, same shape: the graph replays on the new tables
+    seeds2 = seeds + np.uint64(1)
+    _, vs2, _ = dev.batched(sizes, seeds2, thr, mode=0, tile=1024, stage=16, sort=2)
+    assert [int(v) for v in vs2] == _batch_truth(orc, oi, osd, sizes, seeds2, thr)
+
+
+def test_batched_many_batches_global_sums(orc):
+    """more batches than the kernel's LDS table: per-batch sums go to device atomics"""
+    inp, sd, dev = data("small_hash")
+    oi, osd = as_oracle(inp, sd)
+    sizes, seeds, thr = orc.batch_plan(99, 1500, 1, 60)
+    r, vs, done = dev.batched(sizes, seeds, thr, mode=0, tile=2048, stage=16, sort=2)
+    assert [int(v) for v in vs] == _batch_truth(orc, oi, osd, sizes, seeds, thr)
+    assert np.all(done[sizes > 0] > 0)
 
 
 @pytest.mark.parametrize("name", ["small_unionized", "small_hash", "small_nuclide"])
@@ -275,6 +349,14 @@ def test_error_paths_on_device():
         assert e.value.code == 1
         assert dev.event(0)["vsum"] == 0
         assert dev.event(0, kernel_id=X.K_MAT_E_SORT)["vsum"] == 0
+        with pytest.raises(X.XsbError) as e:
+            dev.history(10, 34, kernel_id=3)
+        assert e.value.code == 1
+        assert dev.history(0, 34, kernel_id=6)["vsum"] == 0
+        assert dev.history(10, 0, kernel_id=6)["vsum"] == 0
+        with pytest.raises(X.XsbError) as e:
+            dev.batched([10], [1], np.tile(np.linspace(0.1, 2.0, 12), (1, 1)), mode=3)
+        assert e.value.code == 1
 
 
 # ---------------------------------------------------------------------------

@@ -56,6 +56,26 @@ def test_host_lcg_and_fast_forward_match_reference_vectors():
         assert lib.fast_forward_LCG(int(s), int(n)) == int(out)
 
 
+def test_batch_generator_matches_oracle(orc):
+    """the product's batched-workload generator and per-batch sampling stream
+    equal the oracle's definitions"""
+    sizes, seeds, thr = X.make_batches(2024, 512, 1000, 2_000_000)
+    s2, se2, t2 = orc.batch_plan(2024, 512, 1000, 2_000_000)
+    assert np.array_equal(sizes, s2) and np.array_equal(seeds, se2) and np.array_equal(thr, t2)
+    assert sizes.min() >= 1000 and sizes.max() <= 2_000_000
+    assert np.all(np.diff(thr[:, :11], axis=1) > 0) and np.all(thr[:, 11] == 2.0)
+    # log-uniform: the median size is near sqrt(1e3 * 2e6)
+    assert 2.5e4 < np.median(sizes) < 8e4
+    lib = X.host_lib()
+    e, m = C.c_double(), C.c_int()
+    rng = np.random.default_rng(3)
+    for b in rng.integers(0, 512, 20):
+        t = np.ascontiguousarray(thr[b])
+        for i in (0, 1, int(sizes[b]) - 1, int(rng.integers(0, sizes[b]))):
+            lib.xsbench_batch_sample(int(seeds[b]), t.ctypes.data, i, C.byref(e), C.byref(m))
+            assert (e.value, m.value) == orc.batch_sample(int(seeds[b]), t, i)
+
+
 def test_materials_tables():
     inp = X.make_inputs("large", "hash", "event", gridpoints=4)
     sd = X.grid_init(inp)

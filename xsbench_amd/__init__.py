@@ -115,11 +115,24 @@ class XsbHistoryArgs(C.Structure):
     ]
 
 
+class XsbBatchPlan(C.Structure):
+    _fields_ = [
+        ("tile", C.c_int32), ("stage", C.c_int32), ("sort", C.c_int32), ("windows", C.c_int32),
+        ("tuned", C.c_int32), ("graph", C.c_int32), ("mapping", C.c_int32),
+        ("reserved0", C.c_int32), ("ms_tuning", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class XsbBatchedArgs(C.Structure):
     _fields_ = [
         ("n_batches", C.c_int32), ("xs_precision", C.c_int32), ("sizes", C.c_void_p),
         ("seeds", C.c_void_p), ("thresholds", C.c_void_p), ("mode", C.c_int32),
-        ("tile", C.c_int32), ("out_vsum", C.c_void_p), ("out_done_ms", C.c_void_p),
+        ("tile", C.c_int32), ("stage", C.c_int32), ("sort", C.c_int32), ("window", C.c_int64),
+        ("eager", C.c_int32), ("mapping", C.c_int32),
+        ("out_vsum", C.c_void_p), ("out_done_ms", C.c_void_p), ("out_plan", C.c_void_p),
     ]
 
 
@@ -156,7 +169,8 @@ HOST_SYMBOLS = ("read_CLI", "print_inputs", "print_results", "grid_init_do_not_p
                 "run_history_based_simulation", "run_event_based_simulation_optimization_1",
                 "move_simulation_data_to_device", "release_device_memory", "LCG_random_double",
                 "fast_forward_LCG", "binary_write", "binary_read", "estimate_mem_usage",
-                "xsbench_get_options", "xsbench_last_init_parallel", "xsbench_main")
+                "xsbench_get_options", "xsbench_last_init_parallel", "xsbench_main",
+                "xsbench_make_batches", "xsbench_batch_sample", "run_batched_simulation")
 
 
 # ---------------------------------------------------------------------------
@@ -221,6 +235,13 @@ def host_lib():
         _bind(L, "xsbench_last_init_parallel", C.c_int, [])
         _bind(L, "estimate_mem_usage", C.c_size_t, [Inputs])
         _bind(L, "xsbench_main", C.c_int, [C.c_int, C.POINTER(C.c_char_p)])
+        _bind(L, "xsbench_make_batches", None, [C.c_uint64, C.c_int, C.c_long, C.c_long, C.c_void_p,
+                                                C.c_void_p, C.c_void_p])
+        _bind(L, "xsbench_batch_sample", None, [C.c_uint64, C.c_void_p, C.c_long,
+                                                C.POINTER(C.c_double), C.POINTER(C.c_int)])
+        _bind(L, "run_batched_simulation", C.c_ulonglong,
+              [Inputs, SimulationData, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+               C.c_void_p])
         _host = L
     return _host
 
@@ -232,6 +253,17 @@ def _check(rc):
 
 def _ptr(a):
     return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def make_batches(seed, n_batches, lo, hi):
+    """The batched workload (xsbench_make_batches): per-batch sizes (log-uniform
+    in [lo, hi]), stream seeds and [n_batches, 12] cumulative material thresholds."""
+    sizes = np.empty(n_batches, np.int64)
+    seeds = np.empty(n_batches, np.uint64)
+    thr = np.empty((n_batches, 12), np.float64)
+    host_lib().xsbench_make_batches(seed, n_batches, lo, hi, sizes.ctypes.data, seeds.ctypes.data,
+                                    thr.ctypes.data)
+    return sizes, seeds, thr
 
 
 # ---------------------------------------------------------------------------
@@ -385,7 +417,10 @@ class Device:
         _check(self.lib.xsb_hip_history(self.handle, C.byref(a), C.byref(r)))
         return (r.as_dict(), out) if detail else r.as_dict()
 
-    def batched(self, sizes, seeds, thresholds, precision=XS_FP64, mode=0, tile=0):
+    def batched(self, sizes, seeds, thresholds, precision=XS_FP64, mode=0, tile=0, stage=0, sort=0,
+                window=0, eager=False, mapping=0, plan=None):
+        """xsb_hip_batched; returns (result dict, per-batch vsum, per-batch completion ms).
+        Pass a dict as `plan` to receive the plan that ran (tile, stage, sort, ...)."""
         sizes = np.ascontiguousarray(sizes, np.int64)
         seeds = np.ascontiguousarray(seeds, np.uint64)
         thr = np.ascontiguousarray(thresholds, np.float64)
@@ -400,10 +435,19 @@ class Device:
         a.thresholds = thr.ctypes.data
         a.mode = mode
         a.tile = tile
+        a.stage = stage
+        a.sort = sort
+        a.window = window
+        a.eager = 1 if eager else 0
+        a.mapping = mapping
         a.out_vsum = vs.ctypes.data
         a.out_done_ms = done.ctypes.data
+        pl = XsbBatchPlan()
+        a.out_plan = C.addressof(pl)
         r = XsbResult()
         _check(self.lib.xsb_hip_batched(self.handle, C.byref(a), C.byref(r)))
+        if plan is not None:
+            plan.update(pl.as_dict())
         return r.as_dict(), vs, done
 
     def balg(self, lookups, first=0, precision=XS_FP64, seed=STARTING_SEED):

@@ -109,6 +109,9 @@ static void release(Device *d)
 	(void)hipSetDevice(d->device);
 	if (d->stream)
 		(void)hipStreamSynchronize(d->stream);
+	batch_state_free(d);
+	if (d->history_graph.exec)
+		(void)hipGraphExecDestroy(d->history_graph.exec);
 	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_grid,  d->d_egrid,    d->d_xs32, d->d_xs16,
 	                d->d_xs16_scale, d->d_num_nucs, d->d_mats, d->d_concs, d->d_concs32, d->d_acc,
 	                d->d_fftab};
@@ -450,8 +453,6 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	T.n_iso = (int32_t)n_iso;
 	T.idx_pitch = (int32_t)pitch;
 	T.variant = kVariantDefault;
-	if (const char *v = std::getenv("XSB_LOOKUP_VARIANT"))
-		T.variant = (int32_t)std::strtol(v, nullptr, 0);
 	// per material: the longest run of consecutive ids that starts at a
 	// multiple of 4, cut to a multiple of 4 (read as 16-byte vectors)
 	for (int m = 0; m < kMaterials; m++) {
@@ -708,7 +709,15 @@ int xsb_hip_history(void *handle, const xsb_history_args *a, xsb_result *r)
 		            (long long)a->particles, a->lookups_per_particle);
 	if ((rc = precision_ok(d, a->xs_precision)))
 		return rc;
+	if (a->kernel_id != 0 && a->kernel_id != 6 && a->kernel_id != XSB_K_AUTO)
+		return fail(XSB_ERR_INVALID, "history kernel_id %d is not 0, 6 or -1", a->kernel_id);
 	memset(r, 0, sizeof(*r));
+	// rounds need a lookup to sort on; the automatic choice takes them from
+	// 64K particles on, where a round fills the device
+	const bool rounds = a->particles > 0 && a->lookups_per_particle > 0 &&
+	                    (a->kernel_id == 6 || (a->kernel_id == XSB_K_AUTO && a->particles >= 65536));
+	if (rounds)
+		return run_history_rounds(d, a, r);
 	r->kernel_id = 0;
 	hipStream_t s = d->stream;
 	const int64_t n = a->particles;

@@ -3,20 +3,30 @@
 //
 // Batch b, lookup i:  s = FF(seed_b, 2i), E = LCG(s), roll = LCG(s),
 //                     mat = first m with roll < thr_b[m]  (thr_b[11] >= 1).
+// Global lookup id g = off_b + i numbers all lookups batch after batch.
 //
+// mode 0 (sorted pipeline, replayed as one hipGraph):
+//   k_breset    clears the per-batch sums, counters and queue heads (a fill
+//               kernel: no memset node in the graph)
+//   k_bkeys     key = window << (4 + eb) | mat << eb | top eb bits of E, and
+//               the lookup id, for every lookup of every batch
+//   radix sort  (window, material, energy) order, ids as payload
+//   k_bsorted   persistent lookup kernel: workgroups take tiles of the
+//               sorted order from 8 per-XCD queues (the XCD's own slice of
+//               the order first, so its L2 sees neighbouring energies, then
+//               the other queues); per-batch sums and counts are gathered in
+//               LDS and flushed with one atomic per batch; a batch's
+//               completion time is taken when its count reaches its size.
 // mode 1: one fused launch per batch, in batch order.
-// mode 0: one persistent kernel fed by a device work queue of tiles; each
-//         tile (a slice of one batch) is ordered by material inside the
-//         workgroup before its lookups run, so waves run one material; the
-//         queue reset and the kernel are captured once in a hipGraph and the
-//         graph is replayed.  Each batch records the device clock when its
-//         last tile completes.
+// mode 2: one persistent kernel over a queue of tiles in lookup order; each
+//         tile is ordered by material inside the workgroup.
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
-#include "xsb_internal.h"
-#include "xsb_mixed.h"
+#include "xsb_sorted.h"
 
 namespace xsb {
 
@@ -71,7 +81,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_one(Tables T, int64_t siz
 		block_add_u64<kBatchBlock>(fb, nfb);
 }
 
-// ---- mode 0: persistent kernel over a queue of tiles ----
+// ---- mode 2: persistent kernel over a queue of tiles in lookup order ----
 struct BatchQueue {
 	const int64_t *sizes;       // [nb]
 	const uint64_t *seeds;      // [nb]
@@ -221,23 +231,661 @@ static void launch_one(Device *d, int64_t size, uint64_t seed, const double *thr
 	}
 }
 
-int run_batched(Device *d, const xsb_batched_args *a, xsb_result *r)
+
+// ---- mode 0: sorted pipeline ----
+constexpr int kLdsBatches = 1024;  // per-batch LDS sums when n_batches <= this
+constexpr int kXcds = 8;
+
+// Batch tables on the device.  off[b] = first global id of batch b,
+// off[nb] = total; win[b] = sort window of batch b (non-decreasing).
+struct BatchView {
+	const int64_t *sizes;
+	const uint64_t *seeds;
+	const double *thr;
+	const uint32_t *off;
+	const uint32_t *win;
+	int32_t nb;
+};
+
+// last b with off[b] <= g (off[0] = 0 <= g < off[nb]): the batch holding g,
+// skipping empty batches
+__device__ __forceinline__ int batch_of(const uint32_t *off, int nb, uint32_t g)
 {
-	if (!a || !r || a->n_batches < 1 || !a->sizes || !a->seeds || !a->thresholds)
-		return fail(XSB_ERR_INVALID, "n_batches >= 1 with sizes, seeds and thresholds required");
-	const int nb = a->n_batches;
-	int64_t total = 0;
-	for (int b = 0; b < nb; b++) {
-		if (a->sizes[b] < 0 || a->sizes[b] > ((int64_t)1 << 31))
-			return fail(XSB_ERR_INVALID, "batch %d size %lld out of range", b,
-			            (long long)a->sizes[b]);
-		for (int m = 0; m < kMaterials; m++)
-			if (!(a->thresholds[b * kMaterials + m] >= (m ? a->thresholds[b * kMaterials + m - 1] : 0.0)))
-				return fail(XSB_ERR_INVALID, "batch %d thresholds are not non-decreasing", b);
-		if (!(a->thresholds[b * kMaterials + kMaterials - 1] >= 1.0))
-			return fail(XSB_ERR_INVALID, "batch %d: thresholds[11] must be >= 1", b);
-		total += a->sizes[b];
+	int lo = 0, hi = nb;
+	while (hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if (off[mid] <= g)
+			lo = mid;
+		else
+			hi = mid;
 	}
+	return lo;
+}
+
+struct SortedQueue {
+	const uint32_t *keys;      // sorted keys: material in bits [ebits, ebits + 4)
+	const uint32_t *ids;       // sorted global lookup ids
+	int64_t n;
+	int64_t n_items;
+	int32_t ebits;
+	int32_t tile;
+	int32_t flush_each_item;   // several sort windows: flush LDS sums after every item
+	int32_t wave;              // wave-uniform lookup path for material-uniform waves
+	uint32_t *heads;           // [8] items taken from each XCD's slice
+	unsigned long long *vsum;  // [nb]
+	unsigned long long *cnt;   // [nb] lookups completed
+	uint64_t *done_clock;      // [nb]
+	unsigned long long *nfb;   // [1]
+};
+
+__global__ void k_breset(SortedQueue q, int nb)
+{
+	for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+		q.vsum[b] = 0;
+		q.cnt[b] = 0;
+		q.done_clock[b] = 0;
+	}
+	if (blockIdx.x == 0 && threadIdx.x < kXcds)
+		q.heads[threadIdx.x] = 0;
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		*q.nfb = 0;
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_bkeys(Tables T, BatchView B, int64_t n, int ebits,
+                                                      uint32_t *keys, uint32_t *vals,
+                                                      uint32_t *hist, int ntiles)
+{
+	__shared__ uint32_t h[kBins];
+	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
+		h[d] = 0;
+	__syncthreads();
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int64_t base = (int64_t)blockIdx.x * kTile;
+	const uint32_t emax = (1u << ebits) - 1u;
+	for (int it = 0; it < kSortItems; it++) {
+		const int64_t e = base + tile_pos(wave, it, lane);
+		if (e < n) {
+			const uint32_t g = (uint32_t)e;
+			const int b = batch_of(B.off, B.nb, g);
+			double thr[kMaterials];
+#pragma unroll
+			for (int k = 0; k < kMaterials; k++)
+				thr[k] = B.thr[b * kMaterials + k];
+			double p;
+			int mat;
+			batch_sample(T, B.seeds[b], thr, (int64_t)(g - B.off[b]), p, mat);
+			uint32_t eb = (uint32_t)(p * (double)(1u << ebits));  // exact scaling, truncation
+			eb = eb < emax ? eb : emax;
+			const uint32_t key = (B.win[b] << (4 + ebits)) | ((uint32_t)mat << ebits) | eb;
+			keys[e] = key;
+			vals[e] = g;
+			atomicAdd(&h[key & kDigitMask], 1u);
+		}
+	}
+	__syncthreads();
+	write_tile_hist(h, hist, ntiles);
+}
+
+// Adds the block's LDS sums of batches [0, nb) to the device totals and
+// stamps the completion clock of every batch whose count becomes complete.
+__device__ __forceinline__ void flush_batches(const BatchView &B, const SortedQueue &q,
+                                              uint32_t *s_v, uint32_t *s_c)
+{
+	for (int b = threadIdx.x; b < B.nb; b += kBatchBlock) {
+		const uint32_t c = s_c[b];
+		if (c) {
+			atomicAdd(q.vsum + b, (unsigned long long)s_v[b]);
+			const unsigned long long before = atomicAdd(q.cnt + b, (unsigned long long)c);
+			if (before + c == (unsigned long long)B.sizes[b])
+				q.done_clock[b] = wall_clock64();
+			s_v[b] = 0;
+			s_c[b] = 0;
+		}
+	}
+}
+
+template <int GRID, int PREC, int CHUNK>
+__global__ __launch_bounds__(kBatchBlock) void k_bsorted(Tables T, BatchView B, SortedQueue q)
+{
+	__shared__ uint32_t s_off[kLdsBatches + 1];
+	__shared__ uint32_t s_v[kLdsBatches], s_c[kLdsBatches];
+	__shared__ int s_item;
+	const int nb = B.nb;
+	const bool lds = nb <= kLdsBatches;
+	if (lds) {
+		for (int b = threadIdx.x; b <= nb; b += kBatchBlock)
+			s_off[b] = B.off[b];
+		for (int b = threadIdx.x; b < nb; b += kBatchBlock)
+			s_v[b] = s_c[b] = 0;
+	}
+	__syncthreads();
+	const uint32_t *off = lds ? s_off : B.off;
+	uint32_t fb_total = 0, since_flush = 0;
+	// the dispatcher places workgroup w on XCD w % 8
+	int xq = (int)(blockIdx.x % kXcds);
+	for (int tries = 0; tries < kXcds;) {
+		if (threadIdx.x == 0)
+			s_item = (int)atomicAdd(q.heads + xq, 1u);
+		__syncthreads();
+		const int64_t lo = q.n_items * xq / kXcds, hi = q.n_items * (xq + 1) / kXcds;
+		const int64_t item = lo + s_item;
+		__syncthreads();  // every thread has read s_item before it is rewritten
+		if (item >= hi) {  // this XCD's slice is drained: move on to the next one
+			xq = (xq + 1) % kXcds;
+			tries++;
+			continue;
+		}
+		const int64_t p0 = item * q.tile, p1 = min(p0 + (int64_t)q.tile, q.n);
+		// all threads run the same trip count; lanes past p1 repeat p1 - 1 and
+		// contribute nothing, so waves stay whole for the wave-wide paths
+		for (int64_t r = p0; r < p1; r += kBatchBlock) {
+			const int64_t pos = r + threadIdx.x;
+			const bool valid = pos < p1;
+			const int64_t pp = valid ? pos : p1 - 1;
+			const uint32_t key = q.keys[pp], g = q.ids[pp];
+			const int mat = (int)((key >> q.ebits) & 15u);
+			const int b = batch_of(off, nb, g);
+			uint64_t s = ff_jump(B.seeds[b], 2ULL * (uint64_t)(g - off[b]), T.ff_tab);
+			const double p = lcg_double(s);
+			int64_t idx;
+			double macro[5];
+			uint32_t fb = 0;
+			const int am = sorted_lookup<GRID, PREC, CHUNK>(T, p, mat, q.wave, idx, macro, fb,
+			                                                nullptr);
+			if (valid) {
+				fb_total += fb;
+				if (lds) {
+					atomicAdd(&s_v[b], (uint32_t)am + 1u);
+					atomicAdd(&s_c[b], 1u);
+				} else {
+					atomicAdd(q.vsum + b, (unsigned long long)(am + 1));
+					if (atomicAdd(q.cnt + b, 1ULL) + 1ULL == (unsigned long long)B.sizes[b])
+						q.done_clock[b] = wall_clock64();
+				}
+			}
+		}
+		// LDS sums stay below 2^32: at most 5 per lookup, flushed every 2^26 lookups
+		since_flush += (uint32_t)(p1 - p0);
+		if (lds && (q.flush_each_item || since_flush >= (1u << 26))) {
+			__syncthreads();
+			flush_batches(B, q, s_v, s_c);
+			since_flush = 0;
+		}
+	}
+	if (lds) {
+		__syncthreads();
+		flush_batches(B, q, s_v, s_c);
+	}
+	if (PREC != XSB_XS_FP64)
+		block_add_u64<kBatchBlock>(fb_total, q.nfb);
+}
+
+template <int GRID, int PREC, int CHUNK>
+static int bsorted_occupancy(int *blocks)
+{
+	XSB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_bsorted<GRID, PREC, CHUNK>,
+	                                                     kBatchBlock, 0));
+	return XSB_OK;
+}
+
+template <int GRID, int PREC, int CHUNK>
+static void launch_bsorted(Device *d, const BatchView &B, const SortedQueue &q, unsigned grid,
+                           hipStream_t s)
+{
+	hipLaunchKernelGGL((k_bsorted<GRID, PREC, CHUNK>), dim3(grid), dim3(kBatchBlock), 0, s, d->T, B,
+	                   q);
+}
+
+// dispatch on (grid type, precision, staging depth); the depth only matters
+// for the unionized FP64 wave path, the others use one instantiation
+template <int GRID, int PREC>
+static int dispatch_bsorted(Device *d, const BatchView &B, const SortedQueue &q, int stage,
+                            bool occupancy_only, int *occ, hipStream_t s)
+{
+	const bool deep = !(PREC == XSB_XS_FP64 && GRID == kUnionized) || stage >= 16;
+	if (occupancy_only)
+		return deep ? bsorted_occupancy<GRID, PREC, 16>(occ) : bsorted_occupancy<GRID, PREC, 8>(occ);
+	const unsigned grid = (unsigned)*occ;
+	if (deep)
+		launch_bsorted<GRID, PREC, 16>(d, B, q, grid, s);
+	else
+		launch_bsorted<GRID, PREC, 8>(d, B, q, grid, s);
+	XSB_HIP(hipGetLastError());
+	return XSB_OK;
+}
+
+template <int GRID>
+static int dispatch_bsorted_prec(Device *d, int prec, const BatchView &B, const SortedQueue &q,
+                                 int stage, bool occupancy_only, int *occ, hipStream_t s)
+{
+	if (prec == XSB_XS_FP64)
+		return dispatch_bsorted<GRID, XSB_XS_FP64>(d, B, q, stage, occupancy_only, occ, s);
+	if (prec == XSB_XS_FP32)
+		return dispatch_bsorted<GRID, XSB_XS_FP32>(d, B, q, stage, occupancy_only, occ, s);
+	return dispatch_bsorted<GRID, XSB_XS_BF16>(d, B, q, stage, occupancy_only, occ, s);
+}
+
+// occupancy_only: *occ receives resident workgroups per CU; else *occ is
+// the grid size to launch
+static int run_bsorted(Device *d, int prec, const BatchView &B, const SortedQueue &q, int stage,
+                       bool occupancy_only, int *occ, hipStream_t s)
+{
+	switch (d->T.grid_type) {
+	case kUnionized:
+		return dispatch_bsorted_prec<kUnionized>(d, prec, B, q, stage, occupancy_only, occ, s);
+	case kNuclide:
+		return dispatch_bsorted_prec<kNuclide>(d, prec, B, q, stage, occupancy_only, occ, s);
+	default:
+		return dispatch_bsorted_prec<kHash>(d, prec, B, q, stage, occupancy_only, occ, s);
+	}
+}
+
+// ---- host side of mode 0 ----
+// mapping: 1 one lane per lookup (the reference loop per lane), 2 the
+// wave-uniform path where a wave's lanes share the material
+struct BatchPlan {
+	int tile = 0, stage = 0, sort = 0, mapping = 0;
+	bool fits(const BatchPlan &fixed) const
+	{
+		return (!fixed.tile || tile == fixed.tile) && (!fixed.stage || stage == fixed.stage) &&
+		       (!fixed.sort || sort == fixed.sort) && (!fixed.mapping || mapping == fixed.mapping);
+	}
+	bool complete() const { return tile && stage && sort && mapping; }
+};
+
+// What the tuner's choice is keyed on: the table set, the precision and a
+// coarse histogram of the batch sizes (count, mean and largest, in powers
+// of two), plus the sort windows.
+struct TuneKey {
+	int64_t n_iso;
+	int grid, prec, nb_log, mean_log, max_log, windows;
+	bool operator==(const TuneKey &o) const
+	{
+		return n_iso == o.n_iso && grid == o.grid && prec == o.prec && nb_log == o.nb_log &&
+		       mean_log == o.mean_log && max_log == o.max_log && windows == o.windows;
+	}
+};
+
+// Everything one enqueue of the pipeline needs; its device pointers and
+// sizes are also the key of the captured graph.
+struct PipeShape {
+	int64_t n = 0;
+	int nb = 0, windows = 1, prec = 0, grid_type = 0;
+	BatchPlan plan;
+	int ebits = 0, passes = 1, grid = 0;
+	void *tables = nullptr, *sortbuf = nullptr;
+	bool operator==(const PipeShape &o) const
+	{
+		return n == o.n && nb == o.nb && windows == o.windows && prec == o.prec &&
+		       grid_type == o.grid_type && plan.tile == o.plan.tile &&
+		       plan.stage == o.plan.stage && plan.sort == o.plan.sort &&
+		       plan.mapping == o.plan.mapping && ebits == o.ebits &&
+		       passes == o.passes && grid == o.grid && tables == o.tables && sortbuf == o.sortbuf;
+	}
+};
+
+struct BatchState {
+	std::vector<std::pair<TuneKey, BatchPlan>> tuned;
+	hipGraphExec_t exec = nullptr;
+	PipeShape shape;
+};
+
+void batch_state_free(Device *d)
+{
+	if (!d->batch)
+		return;
+	if (d->batch->exec)
+		(void)hipGraphExecDestroy(d->batch->exec);
+	delete d->batch;
+	d->batch = nullptr;
+}
+
+static int ilog2_ceil(uint64_t x)
+{
+	int l = 0;
+	while (l < 63 && ((uint64_t)1 << l) < x)
+		l++;
+	return l;
+}
+
+// Device layout of the batch tables in scratch slot 1.
+struct TableLayout {
+	size_t sizes, seeds, thr, off, win, vsum, cnt, done, heads, nfb, clk, bytes;
+	explicit TableLayout(int nb)
+	{
+		auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+		sizes = 0;
+		seeds = up(sizes + 8 * (size_t)nb);
+		thr = up(seeds + 8 * (size_t)nb);
+		off = up(thr + 8 * (size_t)kMaterials * nb);
+		win = up(off + 4 * ((size_t)nb + 1));
+		vsum = up(win + 4 * (size_t)nb);
+		cnt = up(vsum + 8 * (size_t)nb);
+		done = up(cnt + 8 * (size_t)nb);
+		heads = up(done + 8 * (size_t)nb);
+		nfb = up(heads + 4 * kXcds);
+		clk = up(nfb + 8);
+		bytes = up(clk + 8 * 4);
+	}
+};
+
+// Enqueues the whole mode-0 pipeline on s (kernels only: capturable).
+static int enqueue_pipeline(Device *d, const PipeShape &ps, const BatchView &B, const SortedQueue &q0,
+                            uint64_t *clk, hipStream_t s)
+{
+	uint32_t *kA = (uint32_t *)ps.sortbuf, *vA = kA + ps.n, *kB = vA + ps.n, *vB = kB + ps.n;
+	uint32_t *hist = vB + ps.n, *part = hist + sort_hist_len(ps.n);
+	const int ntiles = sort_tiles(ps.n);
+	hipLaunchKernelGGL(k_breset, dim3((ps.nb + 255) / 256), dim3(256), 0, s, q0, ps.nb);
+	hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, clk);
+	hipLaunchKernelGGL(k_bkeys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, B, ps.n, ps.ebits, kA, vA,
+	                   hist, ntiles);
+	XSB_HIP(hipGetLastError());
+	int rc = sort_pairs(s, ps.n, ps.passes, true, &kA, &vA, &kB, &vB, hist, part);
+	if (rc)
+		return rc;
+	hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, clk + 1);
+	SortedQueue q = q0;
+	q.keys = kA;
+	q.ids = vA;
+	q.n = ps.n;
+	q.tile = ps.plan.tile;
+	q.n_items = (ps.n + ps.plan.tile - 1) / ps.plan.tile;
+	q.ebits = ps.ebits;
+	q.flush_each_item = ps.windows > 1;
+	q.wave = ps.plan.mapping == 2;
+	int grid = ps.grid;
+	if ((rc = run_bsorted(d, ps.prec, B, q, ps.plan.stage, false, &grid, s)))
+		return rc;
+	hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, clk + 2);
+	XSB_HIP(hipGetLastError());
+	return XSB_OK;
+}
+
+// Shape of one pipeline run over the first nb batches (total n lookups).
+static int make_shape(Device *d, int nb, int64_t n, int windows, int prec, const BatchPlan &plan,
+                      void *tables, PipeShape *ps)
+{
+	ps->n = n;
+	ps->nb = nb;
+	ps->windows = windows;
+	ps->prec = prec;
+	ps->grid_type = d->T.grid_type;
+	ps->plan = plan;
+	const int wbits = windows > 1 ? ilog2_ceil((uint64_t)windows) : 0;
+	if (wbits > 26)
+		return fail(XSB_ERR_INVALID, "%d sort windows is more than 2^26", windows);
+	ps->ebits = plan.sort == 2 ? std::min(16, 26 - wbits) : 0;
+	const int bits = wbits + 4 + ps->ebits;
+	ps->passes = (bits + kDigitBits - 1) / kDigitBits;
+	int occ = 0;
+	int rc = run_bsorted(d, prec, BatchView{}, SortedQueue{}, plan.stage, true, &occ, nullptr);
+	if (rc)
+		return rc;
+	ps->grid = std::max(1, d->cu_count) * std::max(1, occ);
+	ps->grid = (ps->grid + kXcds - 1) / kXcds * kXcds;
+	ps->tables = tables;
+	const size_t sort_bytes =
+	    (size_t)n * 16 + (size_t)sort_hist_len(n) * 4 + (size_t)sort_parts(n) * 4 + 256;
+	return scratch_reserve(d, 2, sort_bytes, &ps->sortbuf);
+}
+
+static void make_views(const PipeShape &ps, char *base, const TableLayout &L, BatchView *B,
+                       SortedQueue *q)
+{
+	B->sizes = (const int64_t *)(base + L.sizes);
+	B->seeds = (const uint64_t *)(base + L.seeds);
+	B->thr = (const double *)(base + L.thr);
+	B->off = (const uint32_t *)(base + L.off);
+	B->win = (const uint32_t *)(base + L.win);
+	B->nb = ps.nb;
+	memset(q, 0, sizeof(*q));
+	q->heads = (uint32_t *)(base + L.heads);
+	q->vsum = (unsigned long long *)(base + L.vsum);
+	q->cnt = (unsigned long long *)(base + L.cnt);
+	q->done_clock = (uint64_t *)(base + L.done);
+	q->nfb = (unsigned long long *)(base + L.nfb);
+}
+
+// Runs the pipeline once, eagerly, over the first nb batches; returns the
+// GPU time in *ms.
+static int time_pipeline(Device *d, int nb, int64_t n, int windows, int prec, const BatchPlan &plan,
+                         char *base, const TableLayout &L, float *ms)
+{
+	PipeShape ps;
+	int rc = make_shape(d, nb, n, windows, prec, plan, base, &ps);
+	if (rc)
+		return rc;
+	BatchView B;
+	SortedQueue q;
+	make_views(ps, base, L, &B, &q);
+	hipStream_t s = d->stream;
+	XSB_HIP(hipEventRecord(d->ev[4], s));
+	if ((rc = enqueue_pipeline(d, ps, B, q, (uint64_t *)(base + L.clk), s)))
+		return rc;
+	XSB_HIP(hipEventRecord(d->ev[5], s));
+	XSB_HIP(hipEventSynchronize(d->ev[5]));
+	XSB_HIP(hipEventElapsedTime(ms, d->ev[4], d->ev[5]));
+	return XSB_OK;
+}
+
+// Picks the plan fields left open in `fixed` by timing each candidate (twice,
+// keeping the faster) on a prefix of the batches of about 16 M lookups; a
+// prefix of a random set of batches keeps its size histogram.  The staging
+// depth and the mapping only matter on the unionized FP64 path.
+static int autotune(Device *d, const xsb_batched_args *a, const std::vector<uint32_t> &off,
+                    const std::vector<uint32_t> &win, char *base, const TableLayout &L,
+                    const BatchPlan &fixed, BatchPlan *plan)
+{
+	const int nb = a->n_batches;
+	int nbt = nb;
+	for (int b = 0; b < nb; b++)
+		if (off[b + 1] >= (16u << 20)) {
+			nbt = b + 1;
+			break;
+		}
+	const int64_t nt = off[nbt];
+	const int wt = (int)win[nbt - 1] + 1;
+	auto pick = [](int v, std::vector<int> all) { return v ? std::vector<int>{v} : all; };
+	const bool staged = d->T.grid_type == kUnionized && a->xs_precision == XSB_XS_FP64;
+	const std::vector<int> tiles = pick(fixed.tile, {2048, 8192});
+	const std::vector<int> sorts = pick(fixed.sort, {2, 1});
+	const std::vector<int> maps = staged ? pick(fixed.mapping, {2, 1}) : pick(fixed.mapping, {1});
+	float best = 1e30f;
+	BatchPlan best_plan;
+	for (int so : sorts)
+		for (int ti : tiles)
+			for (int mp : maps) {
+				const std::vector<int> stages =
+				    (staged && mp == 2) ? pick(fixed.stage, {16, 8}) : pick(fixed.stage, {16});
+				for (int st : stages) {
+					const BatchPlan c{ti, st, so, mp};
+					float ms = 0, ms2 = 0;
+					int rc = time_pipeline(d, nbt, nt, wt, a->xs_precision, c, base, L, &ms);
+					if (!rc)
+						rc = time_pipeline(d, nbt, nt, wt, a->xs_precision, c, base, L, &ms2);
+					if (rc)
+						return rc;
+					ms = std::min(ms, ms2);
+					if (ms < best) {
+						best = ms;
+						best_plan = c;
+					}
+				}
+			}
+	*plan = best_plan;
+	return XSB_OK;
+}
+
+static int run_sorted_batches(Device *d, const xsb_batched_args *a, xsb_result *r)
+{
+	const int nb = a->n_batches;
+	std::vector<uint32_t> off(nb + 1, 0), win(nb, 0);
+	int64_t total = 0, max_size = 0;
+	for (int b = 0; b < nb; b++) {
+		off[b] = (uint32_t)total;
+		total += a->sizes[b];
+		max_size = std::max(max_size, a->sizes[b]);
+	}
+	if (total >= ((int64_t)1 << 32))
+		return fail(XSB_ERR_INVALID, "mode 0 handles fewer than 2^32 lookups per call (%lld)",
+		            (long long)total);
+	off[nb] = (uint32_t)total;
+	int windows = 1;
+	if (a->window > 0) {
+		int64_t last = -1;
+		int w = -1;
+		for (int b = 0; b < nb; b++) {
+			const int64_t k = (int64_t)off[b] / a->window;
+			if (k != last) {
+				w++;
+				last = k;
+			}
+			win[b] = (uint32_t)w;
+		}
+		windows = w + 1;
+	}
+	hipStream_t s = d->stream;
+	if (!d->batch)
+		d->batch = new BatchState();
+	BatchState &st = *d->batch;
+	const TableLayout L(nb);
+	void *mem = nullptr;
+	int rc = scratch_reserve(d, 1, L.bytes, &mem);
+	if (rc)
+		return rc;
+	char *base = (char *)mem;
+	// the batch tables: plain copies before the pipeline, so that the graph
+	// reads them from fixed addresses and replays for any batch contents
+	XSB_HIP(hipMemcpyAsync(base + L.sizes, a->sizes, 8 * (size_t)nb, hipMemcpyHostToDevice, s));
+	XSB_HIP(hipMemcpyAsync(base + L.seeds, a->seeds, 8 * (size_t)nb, hipMemcpyHostToDevice, s));
+	XSB_HIP(hipMemcpyAsync(base + L.thr, a->thresholds, 8 * (size_t)kMaterials * nb,
+	                       hipMemcpyHostToDevice, s));
+	XSB_HIP(hipMemcpyAsync(base + L.off, off.data(), 4 * ((size_t)nb + 1), hipMemcpyHostToDevice, s));
+	XSB_HIP(hipMemcpyAsync(base + L.win, win.data(), 4 * (size_t)nb, hipMemcpyHostToDevice, s));
+	XSB_HIP(hipStreamSynchronize(s));  // host vectors go out of scope with pending copies otherwise
+
+	// plan: the caller's fields, the rest tuned (cached per key)
+	BatchPlan fixed;
+	fixed.tile = a->tile > 0 ? std::max(kBatchBlock, (a->tile + kBatchBlock - 1) / kBatchBlock * kBatchBlock) : 0;
+	fixed.stage = a->stage > 0 ? (a->stage >= 16 ? 16 : 8) : 0;
+	fixed.sort = a->sort > 0 ? (a->sort >= 2 ? 2 : 1) : 0;
+	fixed.mapping = a->mapping > 0 ? (a->mapping >= 2 ? 2 : 1) : 0;
+	BatchPlan plan = fixed;
+	int tuned = 0;
+	double ms_tuning = 0;
+	if (total > 0 && !fixed.complete()) {
+		const TuneKey key{d->T.n_iso, d->T.grid_type, a->xs_precision, ilog2_ceil((uint64_t)nb),
+		                  ilog2_ceil((uint64_t)std::max<int64_t>(1, total / nb)),
+		                  ilog2_ceil((uint64_t)std::max<int64_t>(1, max_size)), windows};
+		bool found = false;
+		for (auto &e : st.tuned)
+			if (e.first == key && e.second.fits(fixed)) {
+				plan = e.second;
+				found = true;
+				break;
+			}
+		if (!found) {
+			const auto t0 = std::chrono::steady_clock::now();
+			if ((rc = autotune(d, a, off, win, base, L, fixed, &plan)))
+				return rc;
+			ms_tuning = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+			tuned = 1;
+			st.tuned.push_back({key, plan});
+		}
+	}
+	if (!plan.tile) plan.tile = 4096;
+	if (!plan.stage) plan.stage = 16;
+	if (!plan.sort) plan.sort = 2;
+	if (!plan.mapping) plan.mapping = 2;
+
+	std::vector<unsigned long long> hv(nb, 0);
+	std::vector<uint64_t> hd(nb, 0);
+	uint64_t clk[4] = {0, 0, 0, 0};
+	unsigned long long hfb = 0;
+	bool used_graph = false;
+	memset(r, 0, sizeof(*r));
+	r->kernel_id = 0;
+	if (total > 0) {
+		PipeShape ps;
+		if ((rc = make_shape(d, nb, total, windows, a->xs_precision, plan, base, &ps)))
+			return rc;
+		BatchView B;
+		SortedQueue q;
+		make_views(ps, base, L, &B, &q);
+		uint64_t *dclk = (uint64_t *)(base + L.clk);
+		if (!a->eager && !(st.exec && st.shape == ps)) {
+			if (st.exec) {
+				(void)hipGraphExecDestroy(st.exec);
+				st.exec = nullptr;
+			}
+			XSB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+			rc = enqueue_pipeline(d, ps, B, q, dclk, s);
+			hipGraph_t graph = nullptr;
+			const hipError_t ec = hipStreamEndCapture(s, &graph);
+			if (rc || ec != hipSuccess) {
+				if (graph)
+					(void)hipGraphDestroy(graph);
+				return rc ? rc : fail(XSB_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+			}
+			const hipError_t ei = hipGraphInstantiate(&st.exec, graph, nullptr, nullptr, 0);
+			(void)hipGraphDestroy(graph);
+			if (ei != hipSuccess) {
+				st.exec = nullptr;
+				return fail(XSB_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+			}
+			st.shape = ps;
+		}
+		XSB_HIP(hipEventRecord(d->ev[0], s));
+		if (a->eager) {
+			if ((rc = enqueue_pipeline(d, ps, B, q, dclk, s)))
+				return rc;
+		} else {
+			XSB_HIP(hipGraphLaunch(st.exec, s));
+			used_graph = true;
+		}
+		XSB_HIP(hipEventRecord(d->ev[3], s));
+		XSB_HIP(hipMemcpyAsync(hv.data(), q.vsum, 8 * (size_t)nb, hipMemcpyDeviceToHost, s));
+		XSB_HIP(hipMemcpyAsync(hd.data(), q.done_clock, 8 * (size_t)nb, hipMemcpyDeviceToHost, s));
+		XSB_HIP(hipMemcpyAsync(clk, dclk, 8 * 3, hipMemcpyDeviceToHost, s));
+		XSB_HIP(hipMemcpyAsync(&hfb, q.nfb, 8, hipMemcpyDeviceToHost, s));
+		XSB_HIP(hipStreamSynchronize(s));
+		float ms = 0;
+		if (hipEventElapsedTime(&ms, d->ev[0], d->ev[3]) == hipSuccess)
+			r->ms_total = ms;
+	}
+	int rate_khz = 0;
+	XSB_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, d->device));
+	const double per_ms = rate_khz > 0 ? 1.0 / rate_khz : 0.0;
+	r->ms_sample = (double)(int64_t)(clk[1] - clk[0]) * per_ms;
+	r->ms_lookup = (double)(int64_t)(clk[2] - clk[1]) * per_ms;
+	uint64_t vs = 0;
+	for (int b = 0; b < nb; b++) {
+		vs += hv[b];
+		if (a->out_vsum)
+			a->out_vsum[b] = hv[b];
+		if (a->out_done_ms)
+			a->out_done_ms[b] = (a->sizes[b] > 0) ? (double)(int64_t)(hd[b] - clk[0]) * per_ms : 0.0;
+	}
+	r->vsum = vs;
+	r->n_fallback = (int64_t)hfb;
+	if (a->out_plan) {
+		a->out_plan->tile = plan.tile;
+		a->out_plan->stage = plan.stage;
+		a->out_plan->sort = plan.sort;
+		a->out_plan->windows = windows;
+		a->out_plan->mapping = plan.mapping;
+		a->out_plan->tuned = tuned;
+		a->out_plan->graph = used_graph ? 1 : 0;
+		a->out_plan->ms_tuning = ms_tuning;
+	}
+	return XSB_OK;
+}
+
+// ---- modes 1 and 2 ----
+static int run_unsorted_batches(Device *d, const xsb_batched_args *a, xsb_result *r)
+{
+	const int nb = a->n_batches;
 	int tile = a->tile > 0 ? a->tile : 4096;
 	tile = ((tile + kBatchBlock - 1) / kBatchBlock) * kBatchBlock;
 	if (tile > kBatchBlock * 16)
@@ -247,7 +895,6 @@ int run_batched(Device *d, const xsb_batched_args *a, xsb_result *r)
 	memset(r, 0, sizeof(*r));
 	r->kernel_id = a->mode;
 
-	// device arrays: sizes, seeds, thr, item_off, vsum, nfb, remaining, done, head, tiles, clock
 	std::vector<int64_t> item_off(nb + 1, 0);
 	std::vector<uint32_t> tiles(nb);
 	for (int b = 0; b < nb; b++) {
@@ -287,14 +934,12 @@ int run_batched(Device *d, const xsb_batched_args *a, xsb_result *r)
 	q.n_items = item_off[nb];
 	q.nb = nb;
 	q.tile = tile;
-	// queue reset with a fill kernel (no memset node in the graph)
 	hipLaunchKernelGGL(k_queue_reset, dim3((nb + 255) / 256), dim3(256), 0, s, q,
 	                   (const uint32_t *)(base + off_tiles));
 	XSB_HIP(hipGetLastError());
 	hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, clk);
 	XSB_HIP(hipEventRecord(d->ev[0], s));
 	if (a->mode == 1) {
-		std::vector<hipEvent_t> evs;
 		for (int b = 0; b < nb; b++) {
 			const double *thr = (const double *)(base + off_thr) + b * kMaterials;
 			if (prec == XSB_XS_FP64)
@@ -334,16 +979,44 @@ int run_batched(Device *d, const xsb_batched_args *a, xsb_result *r)
 		if (a->out_vsum)
 			a->out_vsum[b] = hv[b];
 		if (a->out_done_ms)
-			a->out_done_ms[b] = rate_khz > 0 ? (double)(int64_t)(hd[b] - c0) / rate_khz : 0.0;
+			a->out_done_ms[b] = (rate_khz > 0 && a->sizes[b] > 0)
+			                        ? (double)(int64_t)(hd[b] - c0) / rate_khz
+			                        : 0.0;
 	}
 	r->vsum = vs;
 	r->n_fallback = (int64_t)hfb;
-	r->ms_total = r->ms_lookup = 0;
 	float ms = 0;
 	if (hipEventElapsedTime(&ms, d->ev[0], d->ev[3]) == hipSuccess)
 		r->ms_total = r->ms_lookup = ms;
-	(void)total;
+	if (a->out_plan) {
+		memset(a->out_plan, 0, sizeof(*a->out_plan));
+		a->out_plan->tile = tile;
+	}
 	return XSB_OK;
+}
+
+int run_batched(Device *d, const xsb_batched_args *a, xsb_result *r)
+{
+	if (!a || !r || a->n_batches < 1 || !a->sizes || !a->seeds || !a->thresholds)
+		return fail(XSB_ERR_INVALID, "n_batches >= 1 with sizes, seeds and thresholds required");
+	if (a->n_batches > (1 << 20))
+		return fail(XSB_ERR_INVALID, "n_batches %d is more than 2^20", a->n_batches);
+	if (a->mode < 0 || a->mode > 2)
+		return fail(XSB_ERR_INVALID, "mode %d is not 0, 1 or 2", a->mode);
+	if (a->tile < 0 || a->stage < 0 || a->sort < 0 || a->sort > 2 || a->window < 0)
+		return fail(XSB_ERR_INVALID, "tile, stage, window must be >= 0 and sort 0, 1 or 2");
+	const int nb = a->n_batches;
+	for (int b = 0; b < nb; b++) {
+		if (a->sizes[b] < 0 || a->sizes[b] > ((int64_t)1 << 31))
+			return fail(XSB_ERR_INVALID, "batch %d size %lld out of range", b,
+			            (long long)a->sizes[b]);
+		for (int m = 0; m < kMaterials; m++)
+			if (!(a->thresholds[b * kMaterials + m] >= (m ? a->thresholds[b * kMaterials + m - 1] : 0.0)))
+				return fail(XSB_ERR_INVALID, "batch %d thresholds are not non-decreasing", b);
+		if (!(a->thresholds[b * kMaterials + kMaterials - 1] >= 1.0))
+			return fail(XSB_ERR_INVALID, "batch %d: thresholds[11] must be >= 1", b);
+	}
+	return a->mode == 0 ? run_sorted_batches(d, a, r) : run_unsorted_batches(d, a, r);
 }
 
 }  // namespace xsb

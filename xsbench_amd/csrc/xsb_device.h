@@ -17,11 +17,9 @@ constexpr int kPointDoubles = 6;  // energy + 5 XS channels (NuclideGridPoint)
 
 enum GridType : int { kUnionized = 0, kNuclide = 1, kHash = 2 };
 
-// Lookup-path variants (all give identical results; chosen per table set).
-constexpr int kVariantWaveSearch = 1;   // bracket the wave's energies, short per-lane search
-constexpr int kVariantVecIndex = 2;     // 16-byte index loads over consecutive-id runs
-constexpr int kVariantChunk16 = 4;      // index words fetched 16 (not 32) at a time
-constexpr int kVariantDefault = kVariantVecIndex | kVariantChunk16;
+// Lookup-path choices (all give identical results).
+constexpr int kVariantWave = 1;  // wave-uniform path for material-uniform waves (unionized FP64)
+constexpr int kVariantDefault = kVariantWave;
 
 // Device view of the tables.  Passed by value as a kernel argument.
 struct Tables {
@@ -102,6 +100,15 @@ __device__ __forceinline__ uint64_t ff_jump(uint64_t seed, uint64_t n,
 	return s & kLcgMask;
 }
 
+// Starting skip of history particle p (openmp-threading/Simulation.c:167):
+// p * lookups * 2 * 5 in the reference's 32-bit int arithmetic, sign-extended
+// into the unsigned skip count.
+__device__ __forceinline__ uint64_t history_offset(int64_t p, int lookups)
+{
+	const uint32_t prod = (uint32_t)p * (uint32_t)lookups * 10u;
+	return (uint64_t)(int64_t)(int32_t)prod;
+}
+
 // pick_mat: the reference returns the first i with roll < thr[i], where
 // thr[i] is its reverse-order partial sum (thr[0] = 0), else 0.  The host
 // checks that thr is strictly increasing, so that first i equals the number
@@ -146,35 +153,6 @@ __device__ __forceinline__ int32_t grid_search_range(int32_t lo, int32_t hi, dou
 			lo = mid;
 	}
 	return lo;
-}
-
-__device__ __forceinline__ double wave_min_f64(double x)
-{
-#pragma unroll
-	for (int off = 32; off > 0; off >>= 1)
-		x = fmin(x, __shfl_xor(x, off, 64));
-	return x;
-}
-
-__device__ __forceinline__ double wave_max_f64(double x)
-{
-#pragma unroll
-	for (int off = 32; off > 0; off >>= 1)
-		x = fmax(x, __shfl_xor(x, off, 64));
-	return x;
-}
-
-// grid_search for a full wave (all 64 lanes active): the result is monotone
-// in q, so the brackets of the wave's smallest and largest energy (found
-// once, on wave-uniform values) enclose every lane's result, and each lane
-// bisects only between them.
-__device__ __forceinline__ int32_t grid_search_wave(int32_t n, double q,
-                                                    const double *__restrict__ A)
-{
-	const double qmin = wave_min_f64(q), qmax = wave_max_f64(q);
-	const int32_t L = grid_search_range(0, n - 1, qmin, A);
-	const int32_t H = grid_search_range(L, n - 1, qmax, A);
-	return grid_search_range(L, H + 1, q, A);
 }
 
 // bisection on energies stored with a stride (in doubles)
@@ -302,9 +280,10 @@ __device__ __forceinline__ void accumulate_fp64_select(const double *__restrict_
 		macro[c] += (hi[1 + c] - fr * (hi[1 + c] - lo[1 + c])) * conc;
 }
 
-// One nuclide of the wave-uniform unionized path: bracket index k per lane;
-// the points are read through the scalar cache when the lanes' brackets
-// coincide or differ by one point, else per lane.
+// One nuclide of the wave-uniform unionized path, bracket k per lane.  The
+// points come through the scalar cache when the lanes' brackets coincide
+// (one pair for the wave) or span two neighbouring intervals (three points,
+// each lane picks its pair), else per lane.
 __device__ __forceinline__ void ueg_wave_nuclide(const Tables &T, int nuc, int k, double p,
                                                  double conc, double macro[5])
 {
@@ -324,14 +303,8 @@ __device__ __forceinline__ void ueg_wave_nuclide(const Tables &T, int nuc, int k
 		accumulate_fp64(T.grid + ((int64_t)nuc * n_gp + k) * kPointDoubles, p, conc, macro);
 }
 
-// calculate_macro_xs, unionized grid, FP64, for a wave whose lanes all hold
-// material m (the material-sorted schedules).  The material list, the
-// concentrations and the shared grid points are wave-uniform (scalar cache).
-// Each lane's index-grid words are fetched CHUNK at a time, all in flight
-// together, so a lane's row is read once per chunk instead of once per
-// nuclide; with kVariantVecIndex the material's run of consecutive ids is
-// fetched as 16-byte vectors.  Arithmetic and summation order are those of
-// macro_xs_fp64.
+// Nuclides j0..j1-1 of a material list with scattered ids: each lane's
+// index words are fetched CHUNK at a time, all in flight together.
 template <int CHUNK>
 __device__ __forceinline__ void ueg_wave_scattered(const Tables &T, const int32_t *__restrict__ ml,
                                                    const double *__restrict__ cl,
@@ -351,9 +324,18 @@ __device__ __forceinline__ void ueg_wave_scattered(const Tables &T, const int32_
 		ueg_wave_nuclide(T, ml[j0], row[ml[j0]], p, cl[j0], macro);
 }
 
+// calculate_macro_xs, unionized grid, FP64, for a wave whose lanes all hold
+// material m (the material-sorted schedules).  The material list, the
+// concentrations and the shared grid points are wave-uniform (scalar
+// cache).  A lane's index-grid words are fetched CHUNK (the staging depth,
+// a multiple of 4) at a time, all in flight together, so its row is read
+// once per chunk rather than once per nuclide; over the material's run of
+// consecutive ids (the large fuel's 68..354, 16-byte aligned in the padded
+// rows) they are fetched as 16-byte vectors.  Arithmetic and summation order
+// are those of macro_xs_fp64.
 template <int CHUNK>
-__device__ __forceinline__ void macro_ueg_wave_c(const Tables &T, double p, int m, int64_t idx,
-                                                 double macro[5])
+__device__ __forceinline__ void macro_ueg_wave(const Tables &T, double p, int m, int64_t idx,
+                                               double macro[5])
 {
 #pragma unroll
 	for (int c = 0; c < 5; c++)
@@ -362,10 +344,6 @@ __device__ __forceinline__ void macro_ueg_wave_c(const Tables &T, double p, int 
 	const int32_t *__restrict__ ml = T.mats + m * T.max_num_nucs;
 	const double *__restrict__ cl = T.concs + m * T.max_num_nucs;
 	const int32_t *__restrict__ row = T.index_grid + idx * T.idx_pitch;
-	if (!(T.variant & kVariantVecIndex)) {
-		ueg_wave_scattered<CHUNK>(T, ml, cl, row, 0, nn, p, macro);
-		return;
-	}
 	const int vs = T.vec_start[m], ve = vs + T.vec_len[m];
 	ueg_wave_scattered<CHUNK>(T, ml, cl, row, 0, vs, p, macro);
 	int j0 = vs;
@@ -395,14 +373,7 @@ __device__ __forceinline__ void macro_ueg_wave_c(const Tables &T, double p, int 
 	ueg_wave_scattered<CHUNK>(T, ml, cl, row, ve, nn, p, macro);
 }
 
-__device__ __forceinline__ void macro_ueg_wave(const Tables &T, double p, int m, int64_t idx,
-                                               double macro[5])
-{
-	if (T.variant & kVariantChunk16)
-		macro_ueg_wave_c<16>(T, p, m, idx, macro);
-	else
-		macro_ueg_wave_c<32>(T, p, m, idx, macro);
-}
+constexpr int kIdxChunk = 16;
 
 // argmax with max = -1.0 and strict '>' (Simulation.c:100-110)
 __device__ __forceinline__ int argmax5(const double m[5])

@@ -1,0 +1,266 @@
+// xsb_history.hip -- history-based lookups in rounds (kernel_id 6 of
+// xsb_hip_history; openmp-threading/Simulation.c:116-238).
+//
+// A particle's lookups are a dependent chain, but the particles are
+// independent: round r performs lookup r of every particle.  Each round
+// orders the particles by (material, energy of their next lookup) with the
+// radix sort of the sorted event schedules, then looks them up in that
+// order, so neighbouring lanes read neighbouring grid records, and advances
+// each particle's LCG state exactly as the reference's inner loop does.
+// Particle q's state between rounds: seed[q], the LCG state from which its
+// next energy and material are drawn, and psum[q], its verification sum.
+// The whole chain of rounds is captured once in a hipGraph and replayed.
+#include <cstring>
+#include <vector>
+
+#include "xsb_sorted.h"
+
+namespace xsb {
+
+constexpr int kHBlock = 256;
+constexpr int kHEnergyBits = 16;
+constexpr unsigned kHXcdRun = 16;
+
+__global__ __launch_bounds__(kHBlock) void k_hinit(Tables T, int64_t P, int lookups, uint64_t seed0,
+                                                   uint64_t *seed, uint32_t *psum)
+{
+	const int64_t q = (int64_t)blockIdx.x * kHBlock + threadIdx.x;
+	if (q < P) {
+		seed[q] = ff_jump(seed0, history_offset(q, lookups), T.ff_tab);
+		psum[q] = 0;
+	}
+}
+
+// key = material << 16 | top 16 bits of the next energy; value = particle
+__global__ __launch_bounds__(kSortBlock) void k_hkeys(Tables T, int64_t P,
+                                                      const uint64_t *__restrict__ seed,
+                                                      uint32_t *keys, uint32_t *vals,
+                                                      uint32_t *hist, int ntiles)
+{
+	__shared__ uint32_t h[kBins];
+	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
+		h[d] = 0;
+	__syncthreads();
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int64_t base = (int64_t)blockIdx.x * kTile;
+	for (int it = 0; it < kSortItems; it++) {
+		const int64_t e = base + tile_pos(wave, it, lane);
+		if (e < P) {
+			uint64_t s = seed[e];
+			const double p = lcg_double(s);
+			const int mat = pick_mat(s, T.thr);
+			uint32_t eb = (uint32_t)(p * (double)(1u << kHEnergyBits));
+			eb = eb < (1u << kHEnergyBits) ? eb : (1u << kHEnergyBits) - 1u;
+			const uint32_t key = ((uint32_t)mat << kHEnergyBits) | eb;
+			keys[e] = key;
+			vals[e] = (uint32_t)e;
+			atomicAdd(&h[key & kDigitMask], 1u);
+		}
+	}
+	__syncthreads();
+	write_tile_hist(h, hist, ntiles);
+}
+
+// One round in sorted order (Simulation.c:176-234 for one i): draw the
+// energy and material, look up, add argmax + 1, fast-forward the state by
+// the number of channels above 1.0.  Lanes past P repeat the last particle
+// and write nothing.
+template <int GRID, int PREC, int CHUNK>
+__global__ __launch_bounds__(kHBlock) void k_hround(Tables T, int64_t P,
+                                                    const uint32_t *__restrict__ ids,
+                                                    uint64_t *__restrict__ seed,
+                                                    uint32_t *__restrict__ psum,
+                                                    unsigned long long *vsum,
+                                                    unsigned long long *nfb)
+{
+	const int64_t i = (int64_t)xcd_block(kHXcdRun) * kHBlock + threadIdx.x;
+	const bool valid = i < P;
+	const uint32_t q = ids[valid ? i : P - 1];
+	uint64_t s = seed[q];
+	const double p = lcg_double(s);
+	const int mat = pick_mat(s, T.thr);
+	int64_t idx;
+	double macro[5];
+	uint32_t fb = 0;
+	const int am = sorted_lookup<GRID, PREC, CHUNK, true>(T, p, mat, T.variant & kVariantWave, idx,
+	                                                      macro, fb, nullptr);
+	uint32_t n_forward = 0;
+#pragma unroll
+	for (int c = 0; c < 5; c++)
+		n_forward += (macro[c] > 1.0) ? 1u : 0u;
+	if (n_forward)
+		s = ff_jump(s, n_forward, T.ff_tab);
+	if (valid) {
+		seed[q] = s;
+		psum[q] += (uint32_t)am + 1u;
+	}
+	block_add_u64<kHBlock>(valid ? (uint32_t)am + 1u : 0u, vsum);
+	if (PREC != XSB_XS_FP64)
+		block_add_u64<kHBlock>(valid ? fb : 0u, nfb);
+}
+
+// Per-particle outputs: the sum widened to 64 bits, and the final state
+// after the draws of the energy and material that follow the last lookup
+// (the reference draws them at the end of every iteration).
+__global__ __launch_bounds__(kHBlock) void k_hout(int64_t P, const uint64_t *__restrict__ seed,
+                                                  const uint32_t *__restrict__ psum,
+                                                  uint64_t *out_sum, uint64_t *out_seed)
+{
+	const int64_t q = (int64_t)blockIdx.x * kHBlock + threadIdx.x;
+	if (q < P) {
+		uint64_t s = seed[q];
+		(void)lcg_double(s);
+		(void)lcg_double(s);
+		if (out_sum)
+			out_sum[q] = psum[q];
+		if (out_seed)
+			out_seed[q] = s;
+	}
+}
+
+__global__ void k_hreset(unsigned long long *acc)
+{
+	if (threadIdx.x < 2)
+		acc[threadIdx.x] = 0;
+}
+
+template <int GRID, int PREC>
+static void launch_round(Device *d, hipStream_t s, int64_t P, const uint32_t *ids, uint64_t *seed,
+                         uint32_t *psum, unsigned long long *acc)
+{
+	const unsigned nb = (unsigned)((P + kHBlock - 1) / kHBlock);
+	constexpr int CHUNK = kIdxChunk;
+	hipLaunchKernelGGL((k_hround<GRID, PREC, CHUNK>), dim3(nb), dim3(kHBlock), 0, s, d->T, P, ids,
+	                   seed, psum, acc, acc + 1);
+}
+
+template <int PREC>
+static void launch_round_grid(Device *d, hipStream_t s, int64_t P, const uint32_t *ids,
+                              uint64_t *seed, uint32_t *psum, unsigned long long *acc)
+{
+	switch (d->T.grid_type) {
+	case kUnionized: launch_round<kUnionized, PREC>(d, s, P, ids, seed, psum, acc); break;
+	case kNuclide: launch_round<kNuclide, PREC>(d, s, P, ids, seed, psum, acc); break;
+	default: launch_round<kHash, PREC>(d, s, P, ids, seed, psum, acc); break;
+	}
+}
+
+struct HistoryBuffers {
+	uint64_t *seed;
+	uint32_t *psum, *kA, *vA, *kB, *vB, *hist, *part;
+	uint64_t *out_sum, *out_seed;
+	unsigned long long *acc;
+};
+
+// Enqueues the whole run (kernels only: capturable).
+static int enqueue_rounds(Device *d, hipStream_t s, int64_t P, int lookups, uint64_t seed0, int prec,
+                          const HistoryBuffers &B)
+{
+	const unsigned nb = (unsigned)((P + kHBlock - 1) / kHBlock);
+	const int ntiles = sort_tiles(P);
+	hipLaunchKernelGGL(k_hreset, dim3(1), dim3(64), 0, s, B.acc);
+	hipLaunchKernelGGL(k_hinit, dim3(nb), dim3(kHBlock), 0, s, d->T, P, lookups, seed0, B.seed,
+	                   B.psum);
+	XSB_HIP(hipGetLastError());
+	for (int r = 0; r < lookups; r++) {
+		uint32_t *kA = B.kA, *vA = B.vA, *kB = B.kB, *vB = B.vB;
+		hipLaunchKernelGGL(k_hkeys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, P, B.seed, kA, vA,
+		                   B.hist, ntiles);
+		XSB_HIP(hipGetLastError());
+		int rc = sort_pairs(s, P, 2, true, &kA, &vA, &kB, &vB, B.hist, B.part);
+		if (rc)
+			return rc;
+		if (prec == XSB_XS_FP64)
+			launch_round_grid<XSB_XS_FP64>(d, s, P, vA, B.seed, B.psum, B.acc);
+		else if (prec == XSB_XS_FP32)
+			launch_round_grid<XSB_XS_FP32>(d, s, P, vA, B.seed, B.psum, B.acc);
+		else
+			launch_round_grid<XSB_XS_BF16>(d, s, P, vA, B.seed, B.psum, B.acc);
+		XSB_HIP(hipGetLastError());
+	}
+	if (B.out_sum || B.out_seed) {
+		hipLaunchKernelGGL(k_hout, dim3(nb), dim3(kHBlock), 0, s, P, B.seed, B.psum, B.out_sum,
+		                   B.out_seed);
+		XSB_HIP(hipGetLastError());
+	}
+	return XSB_OK;
+}
+
+int run_history_rounds(Device *d, const xsb_history_args *a, xsb_result *r)
+{
+	const int64_t P = a->particles;
+	const int L = a->lookups_per_particle;
+	hipStream_t s = d->stream;
+	const bool outs = a->out_particle_sum || a->out_final_seed;
+	const size_t bytes = (size_t)P * (8 + 4 + 16) + (size_t)sort_hist_len(P) * 4 +
+	                     (size_t)sort_parts(P) * 4 + (outs ? (size_t)P * 16 : 0) + 1024;
+	void *buf = nullptr;
+	int rc = scratch_reserve(d, 3, bytes, &buf);
+	if (rc)
+		return rc;
+	HistoryBuffers B;
+	char *p = (char *)buf;
+	auto take = [&p](size_t n) {
+		char *q = p;
+		p += (n + 255) & ~(size_t)255;
+		return q;
+	};
+	B.seed = (uint64_t *)take((size_t)P * 8);
+	B.psum = (uint32_t *)take((size_t)P * 4);
+	B.kA = (uint32_t *)take((size_t)P * 4);
+	B.vA = (uint32_t *)take((size_t)P * 4);
+	B.kB = (uint32_t *)take((size_t)P * 4);
+	B.vB = (uint32_t *)take((size_t)P * 4);
+	B.hist = (uint32_t *)take((size_t)sort_hist_len(P) * 4);
+	B.part = (uint32_t *)take((size_t)sort_parts(P) * 4);
+	B.out_sum = a->out_particle_sum ? (uint64_t *)take((size_t)P * 8) : nullptr;
+	B.out_seed = a->out_final_seed ? (uint64_t *)take((size_t)P * 8) : nullptr;
+	B.acc = d->d_acc + 8;
+
+	// the graph is keyed on everything its nodes hold
+	std::vector<int64_t> key = {P, L, (int64_t)a->seed, a->xs_precision, d->T.grid_type,
+	                            (int64_t)(intptr_t)buf, (int64_t)(bool)B.out_sum,
+	                            (int64_t)(bool)B.out_seed};
+	GraphCache &gc = d->history_graph;
+	if (!(gc.exec && gc.key == key)) {
+		if (gc.exec) {
+			(void)hipGraphExecDestroy(gc.exec);
+			gc.exec = nullptr;
+		}
+		XSB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+		rc = enqueue_rounds(d, s, P, L, a->seed, a->xs_precision, B);
+		hipGraph_t graph = nullptr;
+		const hipError_t ec = hipStreamEndCapture(s, &graph);
+		if (rc || ec != hipSuccess) {
+			if (graph)
+				(void)hipGraphDestroy(graph);
+			return rc ? rc : fail(XSB_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+		}
+		const hipError_t ei = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(graph);
+		if (ei != hipSuccess) {
+			gc.exec = nullptr;
+			return fail(XSB_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+		}
+		gc.key = key;
+	}
+	XSB_HIP(hipEventRecord(d->ev[0], s));
+	XSB_HIP(hipGraphLaunch(gc.exec, s));
+	XSB_HIP(hipEventRecord(d->ev[3], s));
+	unsigned long long acc[2];
+	XSB_HIP(hipMemcpyAsync(acc, B.acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+	if (a->out_particle_sum)
+		XSB_HIP(hipMemcpyAsync(a->out_particle_sum, B.out_sum, (size_t)P * 8, hipMemcpyDeviceToHost, s));
+	if (a->out_final_seed)
+		XSB_HIP(hipMemcpyAsync(a->out_final_seed, B.out_seed, (size_t)P * 8, hipMemcpyDeviceToHost, s));
+	XSB_HIP(hipStreamSynchronize(s));
+	r->vsum = acc[0];
+	r->n_fallback = (int64_t)acc[1];
+	float ms = 0;
+	if (hipEventElapsedTime(&ms, d->ev[0], d->ev[3]) == hipSuccess)
+		r->ms_total = r->ms_lookup = ms;
+	r->kernel_id = 6;
+	return XSB_OK;
+}
+
+}  // namespace xsb

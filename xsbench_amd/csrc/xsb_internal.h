@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "xsb_device.h"
 #include "xsb_hip.h"
@@ -27,6 +28,13 @@ int fail(int code, const char *fmt, ...);
 struct Scratch {
 	void *ptr = nullptr;
 	size_t bytes = 0;
+};
+
+// An instantiated hipGraph and the key it was captured for (the sizes,
+// arguments and buffer addresses its nodes hold).
+struct GraphCache {
+	hipGraphExec_t exec = nullptr;
+	std::vector<int64_t> key;
 };
 
 struct Device {
@@ -54,7 +62,12 @@ struct Device {
 	int32_t num_nucs_host[kMaterials] = {0};
 	Scratch scratch[4];
 	hipEvent_t ev[16] = {};
+	struct BatchState *batch = nullptr;  // batched path: graph cache, tuned plans
+	GraphCache history_graph;            // history rounds
 };
+
+// Frees what the batched path keeps on a handle (xsb_batch.hip).
+void batch_state_free(Device *d);
 
 // Grows scratch slot `slot` of `d` to at least `bytes`.
 int scratch_reserve(Device *d, int slot, size_t bytes, void **out);
@@ -88,5 +101,8 @@ int launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);
 // ---- sorted schedules (xsb_sched.hip) ----
 int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, xsb_result *r,
                      unsigned long long *vsum, unsigned long long *nfb);
+
+// ---- history in material-energy sorted rounds (xsb_history.hip) ----
+int run_history_rounds(Device *d, const xsb_history_args *a, xsb_result *r);
 
 }  // namespace xsb

@@ -195,14 +195,8 @@ int launch_lookup_list(Device *d, int64_t n, const double *E, const int32_t *mat
 
 // ---------------------------------------------------------------------------
 // History-based transport, one particle per lane
-// (openmp-threading/Simulation.c:141-237).  The starting offset
-// p * lookups * 2 * 5 follows the reference's 32-bit int arithmetic.
+// (openmp-threading/Simulation.c:141-237).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t history_offset(int64_t p, int lookups)
-{
-	const uint32_t prod = (uint32_t)p * (uint32_t)lookups * 10u;
-	return (uint64_t)(int64_t)(int32_t)prod;
-}
 
 template <int GRID, int PREC>
 __global__ __launch_bounds__(kBlock) void k_history(Tables T, int64_t particles, int lookups,

@@ -3,7 +3,7 @@
 //
 // The samples are never materialised: a key kernel derives each lookup's
 // (material, energy) from its LCG state and emits a sort key plus the lookup
-// id; an LSD radix sort (8-bit digits, stable per-wave ranking with ballots)
+// id; an LSD radix sort (10-bit digits, stable per-wave ranking with ballots)
 // orders the ids; the lookup kernel re-derives (E, mat) from the id.  One
 // launch covers all material segments, so no per-material tail waves.
 //
@@ -11,29 +11,14 @@
 //   k = 3, 5 : key = (material != fuel)             (fuel first)
 //   k = 6    : key = material << 16 | energy bits   (material, then energy)
 // Digits are 10 bits wide, so the material-energy key sorts in two passes.
-#include "xsb_internal.h"
-#include "xsb_mixed.h"
+#include "xsb_sorted.h"
 
 namespace xsb {
 
-constexpr int kSortBlock = 256;
-constexpr int kSortWaves = kSortBlock / 64;
-constexpr int kSortItems = 16;
-constexpr int kTile = kSortBlock * kSortItems;  // 4096 elements per tile
 constexpr int kEnergyBits = 16;
-constexpr int kDigitBits = 10;
-constexpr int kBins = 1 << kDigitBits;
-constexpr uint32_t kDigitMask = kBins - 1;
-constexpr int kScanItems = 16;
-constexpr int kScanChunk = 256 * kScanItems;  // 4096 entries per scan block
-constexpr unsigned kXcdRun = 16;              // lookup blocks per XCD run
+constexpr unsigned kXcdRun = 16;  // lookup blocks per XCD run
 
 enum SortMode : int { kModeMat = 0, kModeFuel = 1, kModeMatE = 2 };
-
-__device__ __forceinline__ int tile_pos(int wave, int item, int lane)
-{
-	return wave * 64 * kSortItems + item * 64 + lane;
-}
 
 __device__ __forceinline__ uint32_t sample_key(const Tables &T, uint64_t seed0, int64_t lookup,
                                                int mode)
@@ -72,11 +57,11 @@ __global__ __launch_bounds__(kSortBlock) void k_keys(Tables T, int64_t first, in
 		}
 	}
 	__syncthreads();
-	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
-		hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
+	write_tile_hist(h, hist, ntiles);
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_hist(const uint32_t *__restrict__ keys, int64_t n,
+template <typename K>
+__global__ __launch_bounds__(kSortBlock) void k_hist(const K *__restrict__ keys, int64_t n,
                                                      int shift, uint32_t *hist, int ntiles)
 {
 	__shared__ uint32_t h[kBins];
@@ -88,11 +73,10 @@ __global__ __launch_bounds__(kSortBlock) void k_hist(const uint32_t *__restrict_
 	for (int it = 0; it < kSortItems; it++) {
 		const int64_t e = base + tile_pos(wave, it, lane);
 		if (e < n)
-			atomicAdd(&h[(keys[e] >> shift) & kDigitMask], 1u);
+			atomicAdd(&h[(uint32_t)(keys[e] >> shift) & kDigitMask], 1u);
 	}
 	__syncthreads();
-	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
-		hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
+	write_tile_hist(h, hist, ntiles);
 }
 
 // ---- exclusive scan of the digit-major histogram (reduce, scan, apply) ----
@@ -191,9 +175,10 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *a, int64_t n,
 // Stable scatter of one digit.  Tile order is (wave, item, lane); each
 // wave ranks its items in that order with ballot-built peer masks and
 // per-wave digit counters in LDS, then waves are offset per digit.
-__global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t *__restrict__ kin,
+template <typename K>
+__global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ kin,
                                                         const uint32_t *__restrict__ vin,
-                                                        uint32_t *__restrict__ kout,
+                                                        K *__restrict__ kout,
                                                         uint32_t *__restrict__ vout, int64_t n,
                                                         int shift,
                                                         const uint32_t *__restrict__ hist,
@@ -207,14 +192,15 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t *__restri
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
 	const int64_t base = (int64_t)blockIdx.x * kTile;
-	uint32_t kk[kSortItems], vv[kSortItems];
+	K kk[kSortItems];
+	uint32_t vv[kSortItems];
 	uint16_t loc[kSortItems];
 	for (int it = 0; it < kSortItems; it++) {
 		const int64_t e = base + tile_pos(wave, it, lane);
 		const bool valid = e < n;
-		kk[it] = valid ? kin[e] : 0u;
+		kk[it] = valid ? kin[e] : (K)0;
 		vv[it] = valid ? vin[e] : 0u;
-		const uint32_t d = (kk[it] >> shift) & kDigitMask;
+		const uint32_t d = (uint32_t)(kk[it] >> shift) & kDigitMask;
 		uint64_t peers = __ballot(valid);
 #pragma unroll
 		for (int b = 0; b < kDigitBits; b++) {
@@ -241,7 +227,7 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const uint32_t *__restri
 	for (int it = 0; it < kSortItems; it++) {
 		const int64_t e = base + tile_pos(wave, it, lane);
 		if (e < n) {
-			const uint32_t d = (kk[it] >> shift) & kDigitMask;
+			const uint32_t d = (uint32_t)(kk[it] >> shift) & kDigitMask;
 			const uint32_t pos = cnt[wave][d] + loc[it];
 			kout[pos] = kk[it];
 			vout[pos] = vv[it];
@@ -277,26 +263,8 @@ __global__ __launch_bounds__(256) void k_lookup_ids(Tables T, int64_t n,
 	double bnd = 0.0;
 	uint32_t fb = 0;
 	int64_t idx;
-	int am;
-	if (PREC == XSB_XS_FP64 && GRID == kUnionized) {
-		const int m0 = __builtin_amdgcn_readfirstlane(mat);
-		if (__ballot(mat != m0) == 0ULL) {
-			idx = (T.variant & kVariantWaveSearch) ? grid_search_wave((int32_t)T.n_union, p, T.ueg)
-			                                       : lookup_index<GRID>(T, p);
-			macro_ueg_wave(T, p, m0, idx, macro);
-		} else {
-			idx = lookup_index<GRID>(T, p);
-			macro_xs_fp64<GRID>(T, p, mat, idx, macro);
-		}
-		am = argmax5(macro);
-	} else if (PREC == XSB_XS_FP64) {
-		idx = lookup_index<GRID>(T, p);
-		macro_xs_fp64<GRID>(T, p, mat, idx, macro);
-		am = argmax5(macro);
-	} else {
-		idx = lookup_index<GRID>(T, p);
-		am = macro_xs_mixed<GRID, PREC>(T, p, mat, idx, macro, &fb, DETAIL ? &bnd : nullptr);
-	}
+	const int am = sorted_lookup<GRID, PREC, kIdxChunk>(T, p, mat, T.variant & kVariantWave, idx,
+	                                                    macro, fb, DETAIL ? &bnd : nullptr);
 	if (!valid)
 		fb = 0;
 	const uint32_t v = valid ? (uint32_t)am + 1u : 0u;
@@ -351,6 +319,47 @@ static int scan_hist(uint32_t *hist, int64_t len, uint32_t *part, hipStream_t s)
 	return XSB_OK;
 }
 
+template <typename K>
+static int sort_pairs_t(hipStream_t s, int64_t n, int passes, bool hist0_ready, K **k, uint32_t **v,
+                        K **k2, uint32_t **v2, uint32_t *hist, uint32_t *part)
+{
+	const int ntiles = sort_tiles(n);
+	const int64_t hlen = sort_hist_len(n);
+	int rc;
+	for (int pass = 0; pass < passes; pass++) {
+		const int shift = kDigitBits * pass;
+		if (pass > 0 || !hist0_ready) {
+			hipLaunchKernelGGL(k_hist<K>, dim3(ntiles), dim3(kSortBlock), 0, s, *k, n, shift, hist,
+			                   ntiles);
+			XSB_HIP(hipGetLastError());
+		}
+		if ((rc = scan_hist(hist, hlen, part, s)))
+			return rc;
+		hipLaunchKernelGGL(k_scatter<K>, dim3(ntiles), dim3(kSortBlock), 0, s, *k, *v, *k2, *v2, n,
+		                   shift, hist, ntiles);
+		XSB_HIP(hipGetLastError());
+		K *t = *k;
+		*k = *k2;
+		*k2 = t;
+		uint32_t *tv = *v;
+		*v = *v2;
+		*v2 = tv;
+	}
+	return XSB_OK;
+}
+
+int sort_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint32_t **k, uint32_t **v,
+               uint32_t **k2, uint32_t **v2, uint32_t *hist, uint32_t *part)
+{
+	return sort_pairs_t<uint32_t>(s, n, passes, hist0_ready, k, v, k2, v2, hist, part);
+}
+
+int sort_pairs64(hipStream_t s, int64_t n, int passes, uint64_t **k, uint32_t **v, uint64_t **k2,
+                 uint32_t **v2, uint32_t *hist, uint32_t *part)
+{
+	return sort_pairs_t<uint64_t>(s, n, passes, false, k, v, k2, v2, hist, part);
+}
+
 int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, xsb_result *r,
                      unsigned long long *vsum, unsigned long long *nfb)
 {
@@ -366,9 +375,9 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 		XSB_HIP(hipEventRecord(d->ev[2], s));
 		return XSB_OK;
 	}
-	const int ntiles = (int)((n + kTile - 1) / kTile);
-	const int64_t hlen = (int64_t)kBins * ntiles;
-	const int nparts = (int)((hlen + kScanChunk - 1) / kScanChunk);
+	const int ntiles = sort_tiles(n);
+	const int64_t hlen = sort_hist_len(n);
+	const int nparts = sort_parts(n);
 	const size_t bytes = (size_t)n * 16 + (size_t)hlen * 4 + (size_t)nparts * 4 + 64 + 4 * kBins;
 	void *buf = nullptr;
 	int rc = scratch_reserve(d, 0, bytes, &buf);
@@ -380,25 +389,8 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	hipLaunchKernelGGL(k_keys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, a->first_lookup, n,
 	                   a->seed, mode, kA, vA, hist, ntiles);
 	XSB_HIP(hipGetLastError());
-	for (int pass = 0; pass < passes; pass++) {
-		const int shift = kDigitBits * pass;
-		if (pass > 0) {
-			hipLaunchKernelGGL(k_hist, dim3(ntiles), dim3(kSortBlock), 0, s, kA, n, shift, hist,
-			                   ntiles);
-			XSB_HIP(hipGetLastError());
-		}
-		if ((rc = scan_hist(hist, hlen, part, s)))
-			return rc;
-		hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(kSortBlock), 0, s, kA, vA, kB, vB, n,
-		                   shift, hist, ntiles);
-		XSB_HIP(hipGetLastError());
-		uint32_t *t = kA;
-		kA = kB;
-		kB = t;
-		t = vA;
-		vA = vB;
-		vB = t;
-	}
+	if ((rc = sort_pairs(s, n, passes, true, &kA, &vA, &kB, &vB, hist, part)))
+		return rc;
 	hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(256), 0, s, hist, ntiles, starts);
 	XSB_HIP(hipGetLastError());
 	XSB_HIP(hipEventRecord(d->ev[1], s));

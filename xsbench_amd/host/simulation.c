@@ -10,6 +10,7 @@
  * un-modded verification sum, like the reference.  Failures print the
  * library's message and exit(1), as the reference's gpuErrchk does.
  */
+#include <math.h>
 #include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -242,6 +243,96 @@ unsigned long long run_history_based_simulation(Inputs in, SimulationData SD, in
 	int rc = xsb_hip_history(h, &a, &r);
 	if (rc)
 		die(rc, "xsb_hip_history");
+	g_opts.last_kernel_ms = r.ms_total;
+	g_opts.last_lookup_ms = r.ms_lookup;
+	g_opts.last_fallbacks = r.n_fallback;
+	if (temporary)
+		xsb_hip_release(h);
+	return r.vsum;
+}
+
+/* ---- batched workload (no reference counterpart) ---- */
+
+/* Batch b: u = LCG draw, size = round(exp(ln lo + u (ln hi - ln lo)))
+ * clamped to [lo, hi]; one draw is skipped and the generator state becomes
+ * the batch's stream seed; then 12 weights 0.05 + U(0,1), normalised into
+ * cumulative thresholds, with thresholds[11] = 2 so that every roll < 1
+ * picks a material. */
+void xsbench_make_batches(uint64_t seed, int n_batches, long lo, long hi, long *sizes,
+                          uint64_t *batch_seeds, double *thresholds)
+{
+	uint64_t s = seed;
+	const double llo = log((double)lo), lhi = log((double)hi);
+	for (int b = 0; b < n_batches; b++) {
+		long size = lround(exp(llo + LCG_random_double(&s) * (lhi - llo)));
+		sizes[b] = size < lo ? lo : (size > hi ? hi : size);
+		(void)LCG_random_double(&s);
+		batch_seeds[b] = s;
+		double w[12], total = 0.0;
+		for (int m = 0; m < 12; m++) {
+			w[m] = 0.05 + LCG_random_double(&s);
+			total += w[m];
+		}
+		double acc = 0.0;
+		for (int m = 0; m < 11; m++) {
+			acc += w[m];
+			thresholds[12 * b + m] = acc / total;
+		}
+		thresholds[12 * b + 11] = 2.0;
+	}
+}
+
+/* Lookup i of a batch: s = FF(batch_seed, 2i), energy = LCG(s), roll = LCG(s),
+ * material = first m with roll < thresholds[m]. */
+void xsbench_batch_sample(uint64_t batch_seed, const double *thresholds, long i, double *energy,
+                          int *mat)
+{
+	uint64_t s = fast_forward_LCG(batch_seed, 2ULL * (uint64_t)i);
+	*energy = LCG_random_double(&s);
+	const double roll = LCG_random_double(&s);
+	int m = 0;
+	while (m < 11 && !(roll < thresholds[m]))
+		m++;
+	*mat = m;
+}
+
+/* All batches on the GPU copy of SD (staged, or a temporary upload) with the
+ * sorted pipeline and the autotuned plan; returns the un-modded sum over all
+ * batches, per-batch sums in out_vsum and completion times in out_done_ms
+ * (both optional). */
+unsigned long long run_batched_simulation(Inputs in, SimulationData SD, int n_batches,
+                                          const long *sizes, const uint64_t *batch_seeds,
+                                          const double *thresholds, unsigned long long *out_vsum,
+                                          double *out_done_ms)
+{
+	void *h = find_staged(SD);
+	int temporary = 0;
+	if (!h) {
+		h = upload(in, SD);
+		temporary = 1;
+	}
+	int64_t *sz = malloc(sizeof(int64_t) * (size_t)(n_batches > 0 ? n_batches : 1));
+	if (!sz) {
+		fprintf(stderr, "out of host memory\n");
+		exit(1);
+	}
+	for (int b = 0; b < n_batches; b++)
+		sz[b] = sizes[b];
+	xsb_batched_args a;
+	memset(&a, 0, sizeof(a));
+	a.n_batches = n_batches;
+	a.xs_precision = g_opts.xs_precision;
+	a.sizes = sz;
+	a.seeds = batch_seeds;
+	a.thresholds = thresholds;
+	a.mode = 0;
+	a.out_vsum = (uint64_t *)out_vsum;
+	a.out_done_ms = out_done_ms;
+	xsb_result r;
+	int rc = xsb_hip_batched(h, &a, &r);
+	free(sz);
+	if (rc)
+		die(rc, "xsb_hip_batched");
 	g_opts.last_kernel_ms = r.ms_total;
 	g_opts.last_lookup_ms = r.ms_lookup;
 	g_opts.last_fallbacks = r.n_fallback;
