@@ -42,6 +42,10 @@ extern "C" {
 /* ---- upload flags ---- */
 #define XSB_BUILD_FP32 0x1u /* also build the FP32 XS table on the device */
 #define XSB_BUILD_BF16 0x2u /* also build the BF16 XS table on the device */
+#define XSB_BUILD_GRIDS 0x4u /* generate the nuclide grid, unionized energies and index grid
+                                on the device (GridInit.c:3-151) instead of copying them:
+                                only n_isotopes, n_gridpoints, grid_type, hash_bins and the
+                                material tables of xsb_tables are read */
 
 /* ---- lookup schedules: the reference's -k kernel ids
  *      (openmp-threading/Main.c:80-93, cuda/Main.cu:62-83) ---- */
@@ -192,6 +196,14 @@ typedef struct {
  * `device` (hipMalloc + hipMemcpy, FP64 AoS + index grid + materials), and
  * build the compact XS tables named by `flags` on the device. */
 int xsb_hip_upload(int device, const xsb_tables *t, uint32_t flags, void **handle);
+
+/* Copies the handle's tables back to host arrays (any may be NULL):
+ * nuclide_grid [n_isotopes x n_gridpoints x 6], unionized_energy_array
+ * [n_union], index_grid [rows x n_isotopes] (unpadded, as the reference lays
+ * it out).  For checking tables built with XSB_BUILD_GRIDS against
+ * grid_init_do_not_profile (openmp-threading/GridInit.c:3). */
+int xsb_hip_download(void *handle, double *nuclide_grid, double *unionized_energy_array,
+                     int32_t *index_grid);
 
 /* cuda/GridInit.cu:81 release_device_memory: free everything the handle owns. */
 int xsb_hip_release(void *handle);

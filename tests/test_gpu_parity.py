@@ -166,6 +166,46 @@ def test_history_rounds_graph_replays():
 
 
 # ---------------------------------------------------------------------------
+# grid construction on the device (GridInit.c:3-151) against the host build,
+# itself byte-identical to the reference (digest fixtures)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["small_unionized", "small_hash", "small_nuclide", "large_hash",
+                                  "xl45k_hash", "large_unionized"])
+def test_device_grid_construction_matches_host(name):
+    inp, sd, _ = data(name)
+    want = X.sd_arrays(inp, sd)
+    print(f"[gpu-test] device grid build {name}", flush=True)
+    with X.Device(inp, None) as dev:
+        got = dev.download()
+        assert np.array_equal(got["nuclide_grid"].reshape(-1, 6).view(np.uint64),
+                              want["nuclide_grid"].view(np.uint64))
+        if inp.grid_type == X.UNIONIZED:
+            assert np.array_equal(got["unionized_energy_array"].view(np.uint64),
+                                  want["unionized_energy_array"].view(np.uint64))
+        if inp.grid_type != X.NUCLIDE:
+            assert np.array_equal(got["index_grid"].reshape(-1), want["index_grid"])
+        del got
+        if name == "large_unionized":
+            assert dev.event(17_000_000, kernel_id=X.K_MAT_E_SORT)["hash"] == 952131
+
+
+def test_device_grid_construction_tiny_grids():
+    """tiny grids (2 and 3 points per nuclide) on a hash and a unionized grid:
+    the device tables equal the host's, including the sweep's rule at the
+    last interval"""
+    for grid, gp in (("unionized", 2), ("unionized", 3), ("hash", 3), ("unionized", 40)):
+        inp = X.make_inputs("small", grid, "event", gridpoints=gp, hash_bins=17)
+        sd = X.grid_init(inp)
+        want = X.sd_arrays(inp, sd)
+        with X.Device(inp, None) as dev:
+            got = dev.download()
+        assert np.array_equal(got["nuclide_grid"].reshape(-1, 6), want["nuclide_grid"])
+        if grid == "unionized":
+            assert np.array_equal(got["unionized_energy_array"], want["unionized_energy_array"])
+        assert np.array_equal(got["index_grid"].reshape(-1), want["index_grid"]), (grid, gp)
+
+
+# ---------------------------------------------------------------------------
 # mixed precision: identical decisions, bounded error
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("prec", [X.XS_FP32, X.XS_BF16])
@@ -197,6 +237,10 @@ def test_mixed_precision_error_per_channel(name, prec, p9999):
           f"{int(mix['fallback'].sum())} of {n} re-evaluated in FP64", flush=True)
     assert np.all(np.quantile(rel, 0.9999, axis=0) < p9999)
     assert np.all(rel.mean(axis=0) < p9999 / 10)
+    if prec == X.XS_FP32:  # certified per lookup and channel
+        assert rel.max() <= 1e-5
+        assert mix["fallback"].mean() < 0.01
+    assert np.all(err[mix["fallback"] == 1] == 0)  # re-evaluated lookups return FP64 sums
 
 
 @pytest.mark.parametrize("prec", [X.XS_FP32, X.XS_BF16])
@@ -278,9 +322,10 @@ def test_batched_per_batch_hashes(orc, mode, prec):
 
 
 @pytest.mark.parametrize("name", ["small_unionized", "small_hash", "small_nuclide"])
-@pytest.mark.parametrize("tile,stage,sort,window", [(256, 8, 1, 0), (2048, 16, 2, 0),
-                                                    (8192, 8, 2, 25000), (512, 16, 1, 1)])
-def test_batched_sorted_plans_identical(orc, name, tile, stage, sort, window):
+@pytest.mark.parametrize("tile,stage,sort,window,mapping", [(256, 8, 1, 0, 2), (2048, 16, 2, 0, 1),
+                                                            (8192, 8, 2, 25000, 2),
+                                                            (512, 16, 1, 1, 1)])
+def test_batched_sorted_plans_identical(orc, name, tile, stage, sort, window, mapping):
     """every plan and sort window of the sorted pipeline gives the oracle's per-batch sums"""
     inp, sd, dev = data(name)
     oi, osd = as_oracle(inp, sd)
@@ -288,9 +333,10 @@ def test_batched_sorted_plans_identical(orc, name, tile, stage, sort, window):
     sizes[3] = 0  # an empty batch
     plan = {}
     r, vs, done = dev.batched(sizes, seeds, thr, mode=0, tile=tile, stage=stage, sort=sort,
-                              window=window, plan=plan)
+                              window=window, mapping=mapping, plan=plan)
     assert [int(v) for v in vs] == _batch_truth(orc, oi, osd, sizes, seeds, thr)
     assert plan["tile"] == tile and plan["sort"] == sort and plan["tuned"] == 0
+    assert plan["mapping"] == mapping
     assert done[3] == 0 and np.all(np.delete(done, 3) > 0)
     if window:
         assert plan["windows"] > 1

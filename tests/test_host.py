@@ -248,3 +248,56 @@ print('rc', X.hip_lib().xsb_hip_upload(0, C.byref(t), 0, C.byref(h)))
     env = dict(os.environ, PYTHONPATH=ROOT, HIP_VISIBLE_DEVICES="-1")
     r = subprocess.run(["python3", "-c", code], capture_output=True, text=True, timeout=120, env=env)
     assert r.returncode == 0 and "rc 2" in r.stdout, r.stdout + r.stderr
+
+
+# ---------------------------------------------------------------------------
+# the restart rule of the device's chunked index-grid construction
+# ---------------------------------------------------------------------------
+def _serial_sweep(E, U):
+    """openmp-threading/GridInit.c:102-119, one nuclide row of E per column"""
+    n_iso, n_gp = E.shape
+    idx = np.zeros(n_iso, np.int64)
+    high = E[:, 1].copy()
+    out = np.zeros((len(U), n_iso), np.int64)
+    for e, u in enumerate(U):
+        for i in range(n_iso):
+            if not (u < high[i]) and idx[i] != n_gp - 2:
+                idx[i] += 1
+                high[i] = E[i, idx[i] + 1]
+            out[e, i] = idx[i]
+    return out
+
+
+def _chunked_sweep(E, U, R):
+    """the device's split: each chunk of R rows restarts from the state at the
+    end of the run of equal energies before it, min(#{k>=1: E_k <= x}, n_gp-2)"""
+    n_iso, n_gp = E.shape
+    out = np.full((len(U), n_iso), -1, np.int64)
+    for e0 in range(0, len(U), R):
+        e1 = min(e0 + R, len(U))
+        for i in range(n_iso):
+            r = e0
+            while r > 0 and U[r - 1] == U[r]:
+                r -= 1
+            idx = 0 if r == 0 else min(int(np.searchsorted(E[i, 1:], U[r - 1], side="right")),
+                                       n_gp - 2)
+            high = E[i, idx + 1]
+            for e in range(r, e1):
+                if not (U[e] < high) and idx != n_gp - 2:
+                    idx += 1
+                    high = E[i, idx + 1]
+                if e >= e0:
+                    out[e, i] = idx
+    return out
+
+
+def test_chunked_index_sweep_equals_serial_sweep():
+    """duplicates within and across nuclides, runs straddling chunk edges,
+    two-point nuclides"""
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        n_iso, n_gp = int(rng.integers(1, 5)), int(rng.integers(2, 9))
+        E = np.sort(rng.integers(0, int(rng.integers(2, 12)), size=(n_iso, n_gp)) / 16.0, axis=1)
+        U = np.sort(E.ravel())
+        R = int(rng.integers(1, 6))
+        assert np.array_equal(_chunked_sweep(E, U, R), _serial_sweep(E, U)), (E, R)

@@ -32,7 +32,7 @@ GRID_TYPES = {"unionized": UNIONIZED, "nuclide": NUCLIDE, "hash": HASH}
 
 XS_FP64, XS_FP32, XS_BF16 = 0, 1, 2
 PRECISIONS = {"fp64": XS_FP64, "fp32": XS_FP32, "bf16": XS_BF16}
-BUILD_FP32, BUILD_BF16 = 0x1, 0x2
+BUILD_FP32, BUILD_BF16, BUILD_GRIDS = 0x1, 0x2, 0x4
 
 # lookup schedules = the reference's -k kernel ids (cuda/Main.cu:62-83)
 K_BASELINE, K_SPLIT, K_MAT_LAUNCH, K_FUEL_LAUNCH, K_MAT_SORT, K_FUEL_SORT, K_MAT_E_SORT = range(7)
@@ -163,7 +163,7 @@ class XsbInfo(C.Structure):
 # C-ABI entry points declared in include/xsb_hip.h (checked by the tests)
 HIP_SYMBOLS = ("xsb_hip_upload", "xsb_hip_release", "xsb_hip_event", "xsb_hip_history",
                "xsb_hip_lookup", "xsb_hip_batched", "xsb_hip_balg", "xsb_hip_copy_probe",
-               "xsb_hip_info", "xsb_hip_strerror", "xsb_hip_last_error")
+               "xsb_hip_info", "xsb_hip_strerror", "xsb_hip_last_error", "xsb_hip_download")
 HOST_SYMBOLS = ("read_CLI", "print_inputs", "print_results", "grid_init_do_not_profile",
                 "load_num_nucs", "load_mats", "load_concs", "run_event_based_simulation",
                 "run_history_based_simulation", "run_event_based_simulation_optimization_1",
@@ -209,6 +209,7 @@ def hip_lib():
         _bind(L, "xsb_hip_balg", C.c_int, [vp, C.POINTER(XsbEventArgs), C.POINTER(C.c_double)])
         _bind(L, "xsb_hip_copy_probe", C.c_int, [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)])
         _bind(L, "xsb_hip_info", C.c_int, [vp, C.POINTER(XsbInfo)])
+        _bind(L, "xsb_hip_download", C.c_int, [vp, vp, vp, vp])
         _bind(L, "xsb_hip_strerror", C.c_char_p, [C.c_int])
         _bind(L, "xsb_hip_last_error", C.c_char_p, [])
         _hip = L
@@ -235,6 +236,9 @@ def host_lib():
         _bind(L, "xsbench_last_init_parallel", C.c_int, [])
         _bind(L, "estimate_mem_usage", C.c_size_t, [Inputs])
         _bind(L, "xsbench_main", C.c_int, [C.c_int, C.POINTER(C.c_char_p)])
+        _bind(L, "load_num_nucs", C.POINTER(C.c_int), [C.c_long])
+        _bind(L, "load_mats", C.POINTER(C.c_int), [C.POINTER(C.c_int), C.c_long, C.POINTER(C.c_int)])
+        _bind(L, "load_concs", C.POINTER(C.c_double), [C.POINTER(C.c_int), C.c_int])
         _bind(L, "xsbench_make_batches", None, [C.c_uint64, C.c_int, C.c_long, C.c_long, C.c_void_p,
                                                 C.c_void_p, C.c_void_p])
         _bind(L, "xsbench_batch_sample", None, [C.c_uint64, C.c_void_p, C.c_long,
@@ -339,14 +343,38 @@ def tables_from_sd(inp, sd):
     return t
 
 
+def material_tables(inp):
+    """xsb_tables holding only the sizes and the material tables (Materials.c:7-116),
+    for a device that generates its grids itself (BUILD_GRIDS)."""
+    L = host_lib()
+    t = XsbTables()
+    t.n_isotopes = inp.n_isotopes
+    t.n_gridpoints = inp.n_gridpoints
+    t.grid_type = inp.grid_type
+    t.hash_bins = inp.hash_bins
+    num_nucs = L.load_num_nucs(inp.n_isotopes)
+    max_n = C.c_int()
+    mats = L.load_mats(num_nucs, inp.n_isotopes, C.byref(max_n))
+    concs = L.load_concs(num_nucs, max_n.value)
+    t.num_nucs = C.cast(num_nucs, C.c_void_p)
+    t.mats = C.cast(mats, C.c_void_p)
+    t.concs = C.cast(concs, C.c_void_p)
+    t.max_num_nucs = max_n.value
+    return t
+
+
 class Device:
-    """A device copy of the tables (one xsb_hip handle, one HIP stream)."""
+    """A device copy of the tables (one xsb_hip handle, one HIP stream).
+    With sd=None the grids are generated on the device (xsb_hip_upload with
+    BUILD_GRIDS): only the material tables come from the host."""
 
     def __init__(self, inp, sd, device=0, build_fp32=False, build_bf16=False):
         self.lib = hip_lib()
         self.inp = inp
-        self.tables = tables_from_sd(inp, sd)
+        self.tables = tables_from_sd(inp, sd) if sd is not None else material_tables(inp)
         flags = (BUILD_FP32 if build_fp32 else 0) | (BUILD_BF16 if build_bf16 else 0)
+        if sd is None:
+            flags |= BUILD_GRIDS
         h = C.c_void_p()
         _check(self.lib.xsb_hip_upload(device, C.byref(self.tables), flags, C.byref(h)))
         self.handle = h
@@ -367,6 +395,20 @@ class Device:
             self.close()
         except Exception:
             pass
+
+    def download(self):
+        """The device tables as host arrays in the reference's layout:
+        nuclide grid [n_iso, n_gp, 6], unionized energies, index grid [rows, n_iso]."""
+        inp = self.inp
+        n = inp.n_isotopes * inp.n_gridpoints
+        grid = np.empty((inp.n_isotopes, inp.n_gridpoints, 6), np.float64)
+        ueg = np.empty(n if inp.grid_type == UNIONIZED else 0, np.float64)
+        rows = n if inp.grid_type == UNIONIZED else (inp.hash_bins if inp.grid_type == HASH else 0)
+        index = np.empty((rows, inp.n_isotopes), np.int32)
+        _check(self.lib.xsb_hip_download(self.handle, grid.ctypes.data,
+                                         ueg.ctypes.data if len(ueg) else None,
+                                         index.ctypes.data if rows else None))
+        return {"nuclide_grid": grid, "unionized_energy_array": ueg, "index_grid": index}
 
     def info(self):
         i = XsbInfo()

@@ -158,8 +158,9 @@ __global__ void k_sorted_check(const double *__restrict__ a, int64_t n, unsigned
 		atomicAdd(bad, (unsigned long long)b);
 }
 
-static int validate_tables(const xsb_tables *t)
+static int validate_tables(const xsb_tables *t, uint32_t flags)
 {
+	const bool gen = (flags & XSB_BUILD_GRIDS) != 0;
 	if (!t)
 		return fail(XSB_ERR_INVALID, "tables is NULL");
 	if (t->n_isotopes < 1 || t->n_gridpoints < 2)
@@ -169,9 +170,9 @@ static int validate_tables(const xsb_tables *t)
 		return fail(XSB_ERR_INVALID, "n_isotopes * n_gridpoints exceeds 2^31-1");
 	if (t->grid_type < 0 || t->grid_type > 2)
 		return fail(XSB_ERR_INVALID, "grid_type %d is not 0, 1 or 2", t->grid_type);
-	if (!t->num_nucs || !t->mats || !t->concs || !t->nuclide_grid)
+	if (!t->num_nucs || !t->mats || !t->concs || (!gen && !t->nuclide_grid))
 		return fail(XSB_ERR_INVALID, "num_nucs, mats, concs and nuclide_grid are required");
-	if (t->length_nuclide_grid != t->n_isotopes * t->n_gridpoints)
+	if (!gen && t->length_nuclide_grid != t->n_isotopes * t->n_gridpoints)
 		return fail(XSB_ERR_INVALID, "length_nuclide_grid %lld != n_isotopes * n_gridpoints",
 		            (long long)t->length_nuclide_grid);
 	if (t->max_num_nucs < 1)
@@ -186,7 +187,13 @@ static int validate_tables(const xsb_tables *t)
 				return fail(XSB_ERR_INVALID, "mats[%d][%d]=%d is not a nuclide id", m, j, nuc);
 		}
 	}
-	if (t->grid_type == XSB_UNIONIZED) {
+	if (gen) {
+		if (t->grid_type == XSB_HASH && t->hash_bins < 1)
+			return fail(XSB_ERR_INVALID, "hash grid needs hash_bins >= 1");
+		if (t->grid_type == XSB_UNIONIZED &&
+		    (double)t->n_isotopes * t->n_gridpoints * t->n_isotopes > 9.0e18 / 4)
+			return fail(XSB_ERR_INVALID, "unionized index grid too large");
+	} else if (t->grid_type == XSB_UNIONIZED) {
 		if (!t->unionized_energy_array || !t->index_grid)
 			return fail(XSB_ERR_INVALID, "unionized grid needs unionized_energy_array and index_grid");
 		if (t->length_unionized_energy_array != t->n_isotopes * t->n_gridpoints)
@@ -252,23 +259,30 @@ __global__ void k_build_bf16(const double *__restrict__ g, int64_t n, int32_t n_
 	}
 }
 
-static int build_compact(Device *d, const xsb_tables *t, uint32_t flags)
+// Per-nuclide max |xs| over the points and channels of host tables.
+static int host_xs_max(const xsb_tables *t, std::vector<double> *mx)
 {
-	const int64_t n_iso = t->n_isotopes, n_gp = t->n_gridpoints, n = n_iso * n_gp;
-	const int max_n = t->max_num_nucs;
-	const int plane = kMaterials * max_n;
-	// per nuclide: max |xs| over points and channels
-	std::vector<double> mx(n_iso, 0.0);
+	const int64_t n_iso = t->n_isotopes, n_gp = t->n_gridpoints;
+	mx->assign((size_t)n_iso, 0.0);
 	for (int64_t nuc = 0; nuc < n_iso; nuc++) {
 		double m = 0.0;
 		const double *p = t->nuclide_grid + nuc * n_gp * kPointDoubles;
 		for (int64_t k = 0; k < n_gp; k++)
 			for (int c = 1; c < kPointDoubles; c++)
 				m = std::fmax(m, std::fabs(p[k * kPointDoubles + c]));
-		if (!std::isfinite(m))
-			return fail(XSB_ERR_INVALID, "nuclide %lld holds non-finite XS", (long long)nuc);
-		mx[nuc] = m;
+		(*mx)[nuc] = m;
 	}
+	return XSB_OK;
+}
+
+static int build_compact(Device *d, const xsb_tables *t, uint32_t flags, const std::vector<double> &mx)
+{
+	const int64_t n_iso = t->n_isotopes, n_gp = t->n_gridpoints, n = n_iso * n_gp;
+	const int max_n = t->max_num_nucs;
+	const int plane = kMaterials * max_n;
+	for (int64_t nuc = 0; nuc < n_iso; nuc++)
+		if (!std::isfinite(mx[nuc]))
+			return fail(XSB_ERR_INVALID, "nuclide %lld holds non-finite XS", (long long)nuc);
 	// power-of-two scale per nuclide with max|xs| <= scale (BF16 table)
 	std::vector<float> scale(n_iso), inv_scale(n_iso);
 	for (int64_t nuc = 0; nuc < n_iso; nuc++) {
@@ -360,9 +374,50 @@ static int upload_pitched(Device *d, const int32_t *host, int64_t rows, int64_t 
 	return XSB_OK;
 }
 
+// rows of `pitch` ints -> rows of n ints
+__global__ void k_unpitch(const int32_t *__restrict__ src, int64_t rows, int32_t n, int32_t pitch,
+                          int32_t *__restrict__ dst)
+{
+	const int64_t total = rows * n;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const int64_t r = i / n;
+		dst[i] = src[r * pitch + (i - r * n)];
+	}
+}
+
+static int download(Device *d, double *grid, double *ueg, int32_t *index)
+{
+	const Tables &T = d->T;
+	const int64_t n = (int64_t)T.n_iso * T.n_gp;
+	if (grid)
+		XSB_HIP(hipMemcpy(grid, d->d_grid, (size_t)n * kPointDoubles * 8, hipMemcpyDeviceToHost));
+	if (ueg && d->d_ueg)
+		XSB_HIP(hipMemcpy(ueg, d->d_ueg, (size_t)T.n_union * 8, hipMemcpyDeviceToHost));
+	if (index && d->d_index) {
+		const int64_t rows = (T.grid_type == kUnionized) ? T.n_union : T.hash_bins;
+		const int64_t stage_rows = std::max<int64_t>(1, ((int64_t)64 << 20) / ((int64_t)T.n_iso * 4));
+		void *stage = nullptr;
+		int rc = scratch_reserve(d, 0, (size_t)std::min(rows, stage_rows) * T.n_iso * 4, &stage);
+		if (rc)
+			return rc;
+		for (int64_t r0 = 0; r0 < rows; r0 += stage_rows) {
+			const int64_t nr = std::min(stage_rows, rows - r0);
+			hipLaunchKernelGGL(k_unpitch, dim3(2048), dim3(256), 0, d->stream,
+			                   d->d_index + r0 * T.idx_pitch, nr, T.n_iso, T.idx_pitch,
+			                   (int32_t *)stage);
+			XSB_HIP(hipGetLastError());
+			XSB_HIP(hipStreamSynchronize(d->stream));
+			XSB_HIP(hipMemcpy(index + r0 * T.n_iso, stage, (size_t)nr * T.n_iso * 4,
+			                  hipMemcpyDeviceToHost));
+		}
+	}
+	return XSB_OK;
+}
+
 static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 {
-	int rc = validate_tables(t);
+	int rc = validate_tables(t, flags);
 	if (rc)
 		return rc;
 	int ndev = 0;
@@ -385,32 +440,51 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 		XSB_HIP(hipEventCreate(&e));
 
 	const int64_t n_iso = t->n_isotopes, n_gp = t->n_gridpoints;
-	if ((rc = upload_array(&d->d_grid, t->nuclide_grid, n_iso * n_gp * kPointDoubles, &d->bytes_fp64)))
-		return rc;
+	const int64_t pitch = (n_iso + 3) & ~(int64_t)3;  // 16-byte aligned rows
+	const bool gen = (flags & XSB_BUILD_GRIDS) != 0;
+	const int64_t n_union = (t->grid_type == XSB_UNIONIZED) ? n_iso * n_gp : 0;
+	const int64_t index_rows = (t->grid_type == XSB_UNIONIZED) ? n_union
+	                           : (t->grid_type == XSB_HASH)    ? t->hash_bins
+	                                                           : 0;
+	{
+		std::vector<uint64_t> tab(8 * 256 * 2);
+		ff_table(tab.data());
+		if ((rc = upload_array(&d->d_fftab, tab.data(), (int64_t)tab.size(), nullptr)))
+			return rc;
+		d->T.ff_tab = d->d_fftab;
+	}
 	if ((rc = upload_array(&d->d_num_nucs, t->num_nucs, kMaterials, nullptr)))
 		return rc;
 	if ((rc = upload_array(&d->d_mats, t->mats, (int64_t)kMaterials * t->max_num_nucs, nullptr)))
 		return rc;
 	if ((rc = upload_array(&d->d_concs, t->concs, (int64_t)kMaterials * t->max_num_nucs, nullptr)))
 		return rc;
-	if (t->grid_type == XSB_UNIONIZED) {
-		if ((rc = upload_array(&d->d_ueg, t->unionized_energy_array, t->length_unionized_energy_array,
-		                       &d->bytes_union)))
+	std::vector<double> xs_max;
+	if (gen) {
+		if ((rc = build_grids_on_device(d, t, pitch, &xs_max)))
 			return rc;
-	}
-	const int64_t pitch = (n_iso + 3) & ~(int64_t)3;  // 16-byte aligned rows
-	if (t->grid_type != XSB_NUCLIDE) {
-		const int64_t rows = t->length_index_grid / n_iso;
-		const size_t bytes = (size_t)rows * pitch * sizeof(int32_t);
-		if (hipMalloc((void **)&d->d_index, bytes) != hipSuccess) {
-			(void)hipGetLastError();
-			d->d_index = nullptr;
-			return fail(XSB_ERR_OOM, "hipMalloc of %zu bytes (index grid) failed", bytes);
+	} else {
+		if ((rc = upload_array(&d->d_grid, t->nuclide_grid, n_iso * n_gp * kPointDoubles,
+		                       &d->bytes_fp64)))
+			return rc;
+		if (t->grid_type == XSB_UNIONIZED) {
+			if ((rc = upload_array(&d->d_ueg, t->unionized_energy_array, n_union, &d->bytes_union)))
+				return rc;
 		}
-		XSB_HIP(hipMemset(d->d_index, 0, bytes));
-		if ((rc = upload_pitched(d, t->index_grid, rows, n_iso, pitch)))
+		if (t->grid_type != XSB_NUCLIDE) {
+			const size_t bytes = (size_t)index_rows * pitch * sizeof(int32_t);
+			if (hipMalloc((void **)&d->d_index, bytes) != hipSuccess) {
+				(void)hipGetLastError();
+				d->d_index = nullptr;
+				return fail(XSB_ERR_OOM, "hipMalloc of %zu bytes (index grid) failed", bytes);
+			}
+			XSB_HIP(hipMemset(d->d_index, 0, bytes));
+			if ((rc = upload_pitched(d, t->index_grid, index_rows, n_iso, pitch)))
+				return rc;
+			d->bytes_index = (int64_t)bytes;
+		}
+		if ((rc = host_xs_max(t, &xs_max)))
 			return rc;
-		d->bytes_index = (int64_t)bytes;
 	}
 	if (hipMalloc((void **)&d->d_acc, 64 * sizeof(unsigned long long)) != hipSuccess) {
 		(void)hipGetLastError();
@@ -424,12 +498,12 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	if (d->d_index) {
 		const int32_t hi = (t->grid_type == XSB_HASH) ? (int32_t)(n_gp - 2) : (int32_t)(n_gp - 1);
 		hipLaunchKernelGGL(k_index_range, dim3(4096), dim3(256), 0, d->stream, d->d_index,
-		                   (int64_t)(t->length_index_grid / n_iso) * pitch, hi, d->d_acc + 60);
+		                   index_rows * pitch, hi, d->d_acc + 60);
 		XSB_HIP(hipGetLastError());
 	}
 	if (d->d_ueg) {
-		hipLaunchKernelGGL(k_sorted_check, dim3(2048), dim3(256), 0, d->stream, d->d_ueg,
-		                   (int64_t)t->length_unionized_energy_array, d->d_acc + 61);
+		hipLaunchKernelGGL(k_sorted_check, dim3(2048), dim3(256), 0, d->stream, d->d_ueg, n_union,
+		                   d->d_acc + 61);
 		XSB_HIP(hipGetLastError());
 	}
 	unsigned long long chk[2] = {0, 0};
@@ -449,7 +523,7 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	T.num_nucs = d->d_num_nucs;
 	T.mats = d->d_mats;
 	T.concs = d->d_concs;
-	T.n_union = (t->grid_type == XSB_UNIONIZED) ? t->length_unionized_energy_array : 0;
+	T.n_union = n_union;
 	T.n_iso = (int32_t)n_iso;
 	T.idx_pitch = (int32_t)pitch;
 	T.variant = kVariantDefault;
@@ -479,13 +553,6 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	T.max_num_nucs = t->max_num_nucs;
 	T.grid_type = t->grid_type;
 	T.hash_du = (t->hash_bins > 0) ? 1.0 / t->hash_bins : 0.0;
-	{
-		std::vector<uint64_t> tab(8 * 256 * 2);
-		ff_table(tab.data());
-		if ((rc = upload_array(&d->d_fftab, tab.data(), (int64_t)tab.size(), nullptr)))
-			return rc;
-		T.ff_tab = d->d_fftab;
-	}
 	pick_thresholds(T.thr);
 	for (int i = 1; i < kMaterials; i++)
 		if (!(T.thr[i] > T.thr[i - 1]))
@@ -493,7 +560,7 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	for (int m = 0; m < kMaterials; m++)
 		d->num_nucs_host[m] = t->num_nucs[m];
 
-	if ((rc = build_compact(d, t, flags)))
+	if ((rc = build_compact(d, t, flags, xs_max)))
 		return rc;
 	T.xs32 = reinterpret_cast<const float *>(d->d_xs32);
 	T.xs16 = d->d_xs16;
@@ -806,6 +873,16 @@ int xsb_hip_copy_probe(void *handle, int64_t bytes, int32_t reps, double *ms_per
 	XSB_HIP(hipStreamSynchronize(s));
 	*ms_per_copy = elapsed(d->ev[4], d->ev[5]) / reps;
 	return XSB_OK;
+}
+
+int xsb_hip_download(void *handle, double *nuclide_grid, double *unionized_energy_array,
+                     int32_t *index_grid)
+{
+	Device *d;
+	int rc = check_handle(handle, &d);
+	if (rc)
+		return rc;
+	return download(d, nuclide_grid, unionized_energy_array, index_grid);
 }
 
 int xsb_hip_info(void *handle, xsb_info *info)

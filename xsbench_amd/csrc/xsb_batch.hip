@@ -14,7 +14,9 @@
 //   k_bsorted   persistent lookup kernel: workgroups take tiles of the
 //               sorted order from 8 per-XCD queues (the XCD's own slice of
 //               the order first, so its L2 sees neighbouring energies, then
-//               the other queues); per-batch sums and counts are gathered in
+//               the other queues; with several sort windows the slices are
+//               dealt out in short runs so windows finish in order); per-batch
+//               sums and counts are gathered in
 //               LDS and flushed with one atomic per batch; a batch's
 //               completion time is taken when its count reaches its size.
 // mode 1: one fused launch per batch, in batch order.
@@ -235,6 +237,7 @@ static void launch_one(Device *d, int64_t size, uint64_t seed, const double *thr
 // ---- mode 0: sorted pipeline ----
 constexpr int kLdsBatches = 1024;  // per-batch LDS sums when n_batches <= this
 constexpr int kXcds = 8;
+constexpr int kRunItems = 4;       // consecutive items per XCD turn with several windows
 
 // Batch tables on the device.  off[b] = first global id of batch b,
 // off[nb] = total; win[b] = sort window of batch b (non-decreasing).
@@ -367,10 +370,20 @@ __global__ __launch_bounds__(kBatchBlock) void k_bsorted(Tables T, BatchView B, 
 		if (threadIdx.x == 0)
 			s_item = (int)atomicAdd(q.heads + xq, 1u);
 		__syncthreads();
-		const int64_t lo = q.n_items * xq / kXcds, hi = q.n_items * (xq + 1) / kXcds;
-		const int64_t item = lo + s_item;
+		// queue xq's k-th item: one slice of the sorted order per XCD, or, with
+		// several sort windows, runs of kRunItems items dealt to the XCDs in
+		// turn, so that all XCDs finish window w before window w + 1
+		const int64_t k = s_item;
+		int64_t item, end;
+		if (q.flush_each_item) {
+			item = (k / kRunItems) * (kXcds * kRunItems) + xq * kRunItems + (k % kRunItems);
+			end = q.n_items;
+		} else {
+			item = q.n_items * xq / kXcds + k;
+			end = q.n_items * (xq + 1) / kXcds;
+		}
 		__syncthreads();  // every thread has read s_item before it is rewritten
-		if (item >= hi) {  // this XCD's slice is drained: move on to the next one
+		if (item >= end) {  // this queue is drained: move on to the next one
 			xq = (xq + 1) % kXcds;
 			tries++;
 			continue;

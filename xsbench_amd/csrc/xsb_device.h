@@ -303,49 +303,34 @@ __device__ __forceinline__ void ueg_wave_nuclide(const Tables &T, int nuc, int k
 		accumulate_fp64(T.grid + ((int64_t)nuc * n_gp + k) * kPointDoubles, p, conc, macro);
 }
 
-// Nuclides j0..j1-1 of a material list with scattered ids: each lane's
-// index words are fetched CHUNK at a time, all in flight together.
-template <int CHUNK>
-__device__ __forceinline__ void ueg_wave_scattered(const Tables &T, const int32_t *__restrict__ ml,
-                                                   const double *__restrict__ cl,
-                                                   const int32_t *__restrict__ row, int j0,
-                                                   int j1, double p, double macro[5])
+// Walks material m's nuclides in list order for the wave-uniform unionized
+// paths, calling f(j, nuclide, bracket k of this lane) for each.  A lane's
+// index-grid words are fetched CHUNK (the staging depth, a multiple of 4) at
+// a time, all in flight together, so its row is read once per chunk rather
+// than once per nuclide; over the material's run of consecutive ids (the
+// large fuel's 68..354, 16-byte aligned in the padded rows) they are fetched
+// as 16-byte vectors.
+template <int CHUNK, class F>
+__device__ __forceinline__ void ueg_wave_rows(const Tables &T, int m, int64_t idx, F &&f)
 {
-	for (; j0 + CHUNK <= j1; j0 += CHUNK) {
-		int kk[CHUNK];
-#pragma unroll
-		for (int u = 0; u < CHUNK; u++)
-			kk[u] = row[ml[j0 + u]];
-#pragma unroll
-		for (int u = 0; u < CHUNK; u++)
-			ueg_wave_nuclide(T, ml[j0 + u], kk[u], p, cl[j0 + u], macro);
-	}
-	for (; j0 < j1; j0++)
-		ueg_wave_nuclide(T, ml[j0], row[ml[j0]], p, cl[j0], macro);
-}
-
-// calculate_macro_xs, unionized grid, FP64, for a wave whose lanes all hold
-// material m (the material-sorted schedules).  The material list, the
-// concentrations and the shared grid points are wave-uniform (scalar
-// cache).  A lane's index-grid words are fetched CHUNK (the staging depth,
-// a multiple of 4) at a time, all in flight together, so its row is read
-// once per chunk rather than once per nuclide; over the material's run of
-// consecutive ids (the large fuel's 68..354, 16-byte aligned in the padded
-// rows) they are fetched as 16-byte vectors.  Arithmetic and summation order
-// are those of macro_xs_fp64.
-template <int CHUNK>
-__device__ __forceinline__ void macro_ueg_wave(const Tables &T, double p, int m, int64_t idx,
-                                               double macro[5])
-{
-#pragma unroll
-	for (int c = 0; c < 5; c++)
-		macro[c] = 0.0;
 	const int nn = T.num_nucs[m];
 	const int32_t *__restrict__ ml = T.mats + m * T.max_num_nucs;
-	const double *__restrict__ cl = T.concs + m * T.max_num_nucs;
 	const int32_t *__restrict__ row = T.index_grid + idx * T.idx_pitch;
 	const int vs = T.vec_start[m], ve = vs + T.vec_len[m];
-	ueg_wave_scattered<CHUNK>(T, ml, cl, row, 0, vs, p, macro);
+	auto scattered = [&](int j0, int j1) {
+		for (; j0 + CHUNK <= j1; j0 += CHUNK) {
+			int kk[CHUNK];
+#pragma unroll
+			for (int u = 0; u < CHUNK; u++)
+				kk[u] = row[ml[j0 + u]];
+#pragma unroll
+			for (int u = 0; u < CHUNK; u++)
+				f(j0 + u, ml[j0 + u], kk[u]);
+		}
+		for (; j0 < j1; j0++)
+			f(j0, ml[j0], row[ml[j0]]);
+	};
+	scattered(0, vs);
 	int j0 = vs;
 	for (; j0 + CHUNK <= ve; j0 += CHUNK) {
 		const int4 *q = reinterpret_cast<const int4 *>(row + ml[j0]);
@@ -355,22 +340,39 @@ __device__ __forceinline__ void macro_ueg_wave(const Tables &T, double p, int m,
 			kk[u] = q[u];
 #pragma unroll
 		for (int u = 0; u < CHUNK / 4; u++) {
-			const int nuc = ml[j0 + 4 * u];
-			ueg_wave_nuclide(T, nuc, kk[u].x, p, cl[j0 + 4 * u], macro);
-			ueg_wave_nuclide(T, nuc + 1, kk[u].y, p, cl[j0 + 4 * u + 1], macro);
-			ueg_wave_nuclide(T, nuc + 2, kk[u].z, p, cl[j0 + 4 * u + 2], macro);
-			ueg_wave_nuclide(T, nuc + 3, kk[u].w, p, cl[j0 + 4 * u + 3], macro);
+			const int j = j0 + 4 * u, nuc = ml[j];
+			f(j, nuc, kk[u].x);
+			f(j + 1, nuc + 1, kk[u].y);
+			f(j + 2, nuc + 2, kk[u].z);
+			f(j + 3, nuc + 3, kk[u].w);
 		}
 	}
 	for (; j0 < ve; j0 += 4) {
 		const int4 k4 = *reinterpret_cast<const int4 *>(row + ml[j0]);
 		const int nuc = ml[j0];
-		ueg_wave_nuclide(T, nuc, k4.x, p, cl[j0], macro);
-		ueg_wave_nuclide(T, nuc + 1, k4.y, p, cl[j0 + 1], macro);
-		ueg_wave_nuclide(T, nuc + 2, k4.z, p, cl[j0 + 2], macro);
-		ueg_wave_nuclide(T, nuc + 3, k4.w, p, cl[j0 + 3], macro);
+		f(j0, nuc, k4.x);
+		f(j0 + 1, nuc + 1, k4.y);
+		f(j0 + 2, nuc + 2, k4.z);
+		f(j0 + 3, nuc + 3, k4.w);
 	}
-	ueg_wave_scattered<CHUNK>(T, ml, cl, row, ve, nn, p, macro);
+	scattered(ve, nn);
+}
+
+// calculate_macro_xs, unionized grid, FP64, for a wave whose lanes all hold
+// material m (the material-sorted schedules).  The material list, the
+// concentrations and the shared grid points are wave-uniform (scalar
+// cache).  Arithmetic and summation order are those of macro_xs_fp64.
+template <int CHUNK>
+__device__ __forceinline__ void macro_ueg_wave(const Tables &T, double p, int m, int64_t idx,
+                                               double macro[5])
+{
+#pragma unroll
+	for (int c = 0; c < 5; c++)
+		macro[c] = 0.0;
+	const double *__restrict__ cl = T.concs + m * T.max_num_nucs;
+	ueg_wave_rows<CHUNK>(T, m, idx, [&](int j, int nuc, int k) {
+		ueg_wave_nuclide(T, nuc, k, p, cl[j], macro);
+	});
 }
 
 constexpr int kIdxChunk = 16;

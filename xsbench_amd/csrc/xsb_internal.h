@@ -102,6 +102,11 @@ int launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);
 int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, xsb_result *r,
                      unsigned long long *vsum, unsigned long long *nfb);
 
+// ---- grid construction on the device (xsb_gridinit.hip): allocates and
+// fills d_grid, d_ueg (unionized) and d_index (rows padded to `pitch`), and
+// returns each nuclide's max |xs| ----
+int build_grids_on_device(Device *d, const xsb_tables *t, int64_t pitch, std::vector<double> *xs_max);
+
 // ---- history in material-energy sorted rounds (xsb_history.hip) ----
 int run_history_rounds(Device *d, const xsb_history_args *a, xsb_result *r);
 

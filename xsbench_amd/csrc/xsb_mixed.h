@@ -19,7 +19,13 @@
 //   B = 1.01 * A * (u + 12 u32),   A = sum_j conc_j M_j (1 + 2|f_j|)
 // (the 1.01 absorbs the FP32 evaluation of A itself and the rounding of
 // conc_j M_j, for n < 10^5).  A lookup whose argmax gap is <= 2B, or (history
-// mode) with some |S_c - 1| <= B, is recomputed in FP64.
+// mode) with some |S_c - 1| <= B, is recomputed in FP64, and so is, with
+// FP32 tables, a lookup with some channel where B > kRelTol |S_c| (sums whose
+// terms nearly cancel, from extrapolation below a nuclide's first point):
+// every FP32 result is then within relative error kRelTol of the FP64
+// reference, channel by channel.  B is a worst case over the term signs and
+// rounding directions (about 8e-7 A); the errors actually seen are ~3e-8
+// relative on average.  A recomputed lookup returns the FP64 sums.
 #pragma once
 
 #include "xsb_device.h"
@@ -27,6 +33,7 @@
 namespace xsb {
 
 constexpr int kRec32Doubles = 4;  // 32 B record
+constexpr double kRelTol = 1e-5;  // FP32 tables: certified relative error per channel
 constexpr int kRec16Doubles = 3;  // 24 B record
 
 // Per (material, position) constants for the mixed kernels, [12][max_num_nucs]
@@ -79,45 +86,32 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float t)
 	sum = s;
 }
 
-// Returns the argmax the FP64 reference would return; macro[] receives the
-// mixed-precision sums (or the FP64 sums when the lookup fell back, which
-// HISTORY mode needs for its > 1.0 test).  *fell_back is set to 1 on fallback.
-template <int GRID, int PREC, bool HISTORY = false>
-__device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat, int64_t idx,
-                                              double macro[5], uint32_t *fell_back,
-                                              double *bound = nullptr)
+// One nuclide's term: interpolate the 5 channels between (e_lo, xl) and
+// (e_hi, xh) with the FP64 fraction, add x * c to the Kahan sums, and c_bound
+// (1 + 2|f|) to the bound accumulator A.
+__device__ __forceinline__ void mixed_term(double e_lo, const float xl[5], double e_hi,
+                                           const float xh[5], double p, float c, float cb,
+                                           float sum[5], float comp[5], float &A)
 {
-	constexpr int REC = (PREC == XSB_XS_FP32) ? kRec32Doubles : kRec16Doubles;
-	constexpr float kU = (PREC == XSB_XS_FP32) ? 0x1p-24f : 0x1p-8f;
-	const int nn = T.num_nucs[mat];
-	const int plane = T.max_num_nucs * kMaterials;
-	const float *__restrict__ ceff = T.concs32 + (PREC == XSB_XS_FP32 ? 0 : 2) * plane +
-	                                 mat * T.max_num_nucs;
-	const float *__restrict__ cbnd = ceff + plane;
-	const int32_t *__restrict__ ml = T.mats + mat * T.max_num_nucs;
-	float sum[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-	float A = 0.f;
-	for (int j = 0; j < nn; j++) {
-		const int nuc = ml[j];
-		const double *row = mixed_row<PREC>(T, nuc);
-		const int32_t lower = nuclide_lower<GRID, REC>(T, row, nuc, idx, p);
-		const double *lo = row + (int64_t)lower * REC;
-		const double *hi = lo + REC;
-		const double e_lo = lo[0], e_hi = hi[0];
-		float xl[5], xh[5];
-		load_xs<PREC>(lo, xl);
-		load_xs<PREC>(hi, xh);
-		const double fr64 = (e_hi - p) / (e_hi - e_lo);
-		const float fr = (float)fr64;
-		const float c = ceff[j];
+	const double fr64 = (e_hi - p) / (e_hi - e_lo);
+	const float fr = (float)fr64;
 #pragma unroll
-		for (int k = 0; k < 5; k++) {
-			const float x = __builtin_fmaf(-fr, xh[k] - xl[k], xh[k]);
-			kahan_add(sum[k], comp[k], x * c);
-		}
-		A += cbnd[j] * (1.0f + 2.0f * fabsf(fr));
+	for (int k = 0; k < 5; k++) {
+		const float x = __builtin_fmaf(-fr, xh[k] - xl[k], xh[k]);
+		kahan_add(sum[k], comp[k], x * c);
 	}
+	A += cb * (1.0f + 2.0f * fabsf(fr));
+}
+
+// The decision on the mixed sums (see the bound above): returns the argmax
+// the FP64 reference would return; macro[] receives the mixed-precision sums,
+// or the FP64 sums when the lookup fell back (*fell_back = 1).
+template <int GRID, int PREC, bool HISTORY>
+__device__ __forceinline__ int mixed_finish(const Tables &T, double p, int mat, int64_t idx,
+                                            const float sum[5], float A, double macro[5],
+                                            uint32_t *fell_back, double *bound)
+{
+	constexpr float kU = (PREC == XSB_XS_FP32) ? 0x1p-24f : 0x1p-8f;
 	// argmax of the mixed sums (max = -1.0, strict '>', as the reference)
 	double s[5];
 #pragma unroll
@@ -133,6 +127,9 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 		if (k != am && !(s[am] - s[k] > 2.0 * B))
 			ambiguous = true;
 		if (HISTORY && fabs(s[k] - 1.0) <= B)
+			ambiguous = true;
+		// B <= kRelTol |S_c| certifies |S_c - R_c| <= kRelTol |S_c| / (1 - kRelTol)
+		if (PREC == XSB_XS_FP32 && !(B <= kRelTol * (1.0 - kRelTol) * fabs(s[k])))
 			ambiguous = true;
 	}
 	// the reference's argmax needs max > -1.0 for a non-zero index; every
@@ -150,12 +147,108 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 	*fell_back = 1;
 	double m64[5];
 	macro_xs_fp64<GRID>(T, p, mat, idx, m64);
-	if (HISTORY) {
 #pragma unroll
-		for (int k = 0; k < 5; k++)
-			macro[k] = m64[k];
-	}
+	for (int k = 0; k < 5; k++)
+		macro[k] = m64[k];
 	return argmax5(m64);
+}
+
+template <int PREC>
+__device__ __forceinline__ const float *mixed_consts(const Tables &T, int mat)
+{
+	const int plane = T.max_num_nucs * kMaterials;
+	return T.concs32 + (PREC == XSB_XS_FP32 ? 0 : 2) * plane + mat * T.max_num_nucs;
+}
+
+// calculate_macro_xs on the mixed tables, one lane per lookup.
+template <int GRID, int PREC, bool HISTORY = false>
+__device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat, int64_t idx,
+                                              double macro[5], uint32_t *fell_back,
+                                              double *bound = nullptr)
+{
+	constexpr int REC = (PREC == XSB_XS_FP32) ? kRec32Doubles : kRec16Doubles;
+	const int nn = T.num_nucs[mat];
+	const float *__restrict__ ceff = mixed_consts<PREC>(T, mat);
+	const float *__restrict__ cbnd = ceff + T.max_num_nucs * kMaterials;
+	const int32_t *__restrict__ ml = T.mats + mat * T.max_num_nucs;
+	float sum[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+	float A = 0.f;
+	for (int j = 0; j < nn; j++) {
+		const int nuc = ml[j];
+		const double *row = mixed_row<PREC>(T, nuc);
+		const int32_t lower = nuclide_lower<GRID, REC>(T, row, nuc, idx, p);
+		const double *lo = row + (int64_t)lower * REC;
+		const double *hi = lo + REC;
+		float xl[5], xh[5];
+		load_xs<PREC>(lo, xl);
+		load_xs<PREC>(hi, xh);
+		mixed_term(lo[0], xl, hi[0], xh, p, ceff[j], cbnd[j], sum, comp, A);
+	}
+	return mixed_finish<GRID, PREC, HISTORY>(T, p, mat, idx, sum, A, macro, fell_back, bound);
+}
+
+// One nuclide of the wave-uniform unionized path on the mixed tables,
+// bracket k per lane: records through the scalar cache when the lanes'
+// brackets coincide or span two neighbouring intervals, else per lane.
+template <int PREC>
+__device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc, int k, double p,
+                                                       float c, float cb, float sum[5],
+                                                       float comp[5], float &A)
+{
+	constexpr int REC = (PREC == XSB_XS_FP32) ? kRec32Doubles : kRec16Doubles;
+	const int32_t n_gp = T.n_gp;
+	if (k == n_gp - 1)
+		k--;
+	const int k0 = __builtin_amdgcn_readfirstlane(k);
+	const int d = k - k0;
+	const double *row = mixed_row<PREC>(T, nuc);
+	const double *P = row + (int64_t)k0 * REC;
+	float x0[5], x1[5];
+	if (__ballot(d != 0) == 0ULL) {
+		load_xs<PREC>(P, x0);
+		load_xs<PREC>(P + REC, x1);
+		mixed_term(P[0], x0, P[REC], x1, p, c, cb, sum, comp, A);
+	} else if (__ballot(d != 0 && d != 1) == 0ULL || __ballot(d != 0 && d != -1) == 0ULL) {
+		// three records Q, Q+1, Q+2 shared; each lane takes its pair
+		const double *Q = (__ballot(d == -1) != 0ULL) ? P - REC : P;
+		const int sel = (__ballot(d == -1) != 0ULL) ? d + 1 : d;
+		float x2[5];
+		load_xs<PREC>(Q, x0);
+		load_xs<PREC>(Q + REC, x1);
+		load_xs<PREC>(Q + 2 * REC, x2);
+		const double e0 = Q[0], e1 = Q[REC], e2 = Q[2 * REC];
+		float xl[5], xh[5];
+#pragma unroll
+		for (int w = 0; w < 5; w++) {
+			xl[w] = sel ? x1[w] : x0[w];
+			xh[w] = sel ? x2[w] : x1[w];
+		}
+		mixed_term(sel ? e1 : e0, xl, sel ? e2 : e1, xh, p, c, cb, sum, comp, A);
+	} else {
+		const double *L = row + (int64_t)k * REC;
+		load_xs<PREC>(L, x0);
+		load_xs<PREC>(L + REC, x1);
+		mixed_term(L[0], x0, L[REC], x1, p, c, cb, sum, comp, A);
+	}
+}
+
+// calculate_macro_xs, unionized grid, mixed tables, for a wave whose lanes
+// all hold material m; same terms and decision as macro_xs_mixed.
+template <int PREC, int CHUNK, bool HISTORY>
+__device__ __forceinline__ int macro_ueg_wave_mixed(const Tables &T, double p, int m, int64_t idx,
+                                                    double macro[5], uint32_t *fell_back,
+                                                    double *bound)
+{
+	const float *__restrict__ ceff = mixed_consts<PREC>(T, m);
+	const float *__restrict__ cbnd = ceff + T.max_num_nucs * kMaterials;
+	float sum[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+	float A = 0.f;
+	ueg_wave_rows<CHUNK>(T, m, idx, [&](int j, int nuc, int k) {
+		ueg_wave_nuclide_mixed<PREC>(T, nuc, k, p, ceff[j], cbnd[j], sum, comp, A);
+	});
+	return mixed_finish<kUnionized, PREC, HISTORY>(T, p, m, idx, sum, A, macro, fell_back, bound);
 }
 
 }  // namespace xsb

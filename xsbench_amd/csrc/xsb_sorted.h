@@ -50,10 +50,10 @@ __device__ __forceinline__ void write_tile_hist(const uint32_t *h, uint32_t *his
 }
 
 // ---- lookup body in sorted order ----
-// One lane's lookup (p, mat).  Unionized FP64 waves whose lanes share the
-// material take the wave-uniform path (macro_ueg_wave, index words staged
-// CHUNK deep) when `wave` is set; everything else runs the per-lane
-// reference loop.  Returns the argmax channel; idx gets the grid-level index,
+// One lane's lookup (p, mat).  Unionized waves whose lanes share the
+// material take the wave-uniform path (macro_ueg_wave / _mixed, index words
+// staged CHUNK deep) when `wave` is set; everything else runs the per-lane
+// loop.  Returns the argmax channel; idx gets the grid-level index,
 // macro the sums, fb the fallback flag (mixed precision).
 // HISTORY: the caller also tests macro_c > 1.0, so mixed precision falls
 // back to FP64 near 1.0 too and then returns the FP64 sums.
@@ -73,6 +73,10 @@ __device__ __forceinline__ int sorted_lookup(const Tables &T, double p, int mat,
 	} else if (PREC == XSB_XS_FP64) {
 		macro_xs_fp64<GRID>(T, p, mat, idx, macro);
 		return argmax5(macro);
+	} else if (GRID == kUnionized) {
+		const int m0 = __builtin_amdgcn_readfirstlane(mat);
+		if (wave && __ballot(mat != m0) == 0ULL)
+			return macro_ueg_wave_mixed<PREC, CHUNK, HISTORY>(T, p, m0, idx, macro, &fb, bnd);
 	}
 	return macro_xs_mixed<GRID, PREC, HISTORY>(T, p, mat, idx, macro, &fb, bnd);
 }

@@ -1,13 +1,15 @@
-"""Fixed workload for hardware-counter runs (rocprofv3 --pmc).
+"""Fixed workload for rocprofv3 runs of the lookup path.
 
-Runs the lookup path a few times on one workload plus a plain device copy of
-known size, so that per-dispatch counters of the lookup kernel can be scaled
-by what the same counters read for the copy (bytes moved = 2 x copy size).
+Runs one workload a few times plus a plain device copy of known size, so that
+per-dispatch hardware counters of the lookup kernel can be scaled by what the
+same counters read for the copy (which moves exactly 2 x COPY_BYTES), and
+writes what the summary needs (algorithmic bytes, sizes) as JSON.
 
-    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \
-        python3 -m xsbench_amd.profile_driver --workload large_unionized --kernel 6
+    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \\
+        python3 xsbench_amd/profile_driver.py --workload large_unionized --kernel 6 --meta OUT/meta.json
 """
 import argparse
+import json
 import os
 import sys
 
@@ -32,23 +34,34 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=sorted(X.PRECISIONS))
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--lookups", type=int, default=17_000_000)
-    ap.add_argument("--particles", type=int, default=100_000)
+    ap.add_argument("--particles", type=int, default=1_000_000)
+    ap.add_argument("--meta", default=None, help="write workload facts as JSON here")
     args = ap.parse_args()
     size, grid, method, gp = WORKLOADS[args.workload]
     inp = X.make_inputs(size, grid, "event", gridpoints=gp)
     sd = X.grid_init(inp)
     prec = X.PRECISIONS[args.precision]
     dev = X.Device(inp, sd, build_fp32=prec == X.XS_FP32, build_bf16=prec == X.XS_BF16)
+    ms = []
     for _ in range(args.reps):
         if method == "event":
             r = dev.event(args.lookups, 0, kernel_id=args.kernel, precision=prec)
         else:
             r = dev.history(args.particles, 34, kernel_id=args.kernel, precision=prec)
+        ms.append(r["ms_lookup"])
         print(f"[profile] {args.workload} k={args.kernel} {args.precision}: {r['ms_total']:.3f} ms, "
               f"lookup kernels {r['ms_lookup']:.3f} ms, hash {r['hash']}", flush=True)
-    ms = dev.copy_probe(COPY_BYTES, reps=args.reps)
-    print(f"[profile] copy of {COPY_BYTES} bytes: {ms:.3f} ms ({2 * COPY_BYTES / ms / 1e6:.0f} GB/s)",
-          flush=True)
+    copy_ms = dev.copy_probe(COPY_BYTES, reps=args.reps)
+    print(f"[profile] copy of {COPY_BYTES} bytes: {copy_ms:.3f} ms "
+          f"({2 * COPY_BYTES / copy_ms / 1e6:.0f} GB/s)", flush=True)
+    if args.meta:
+        meta = {"workload": args.workload, "kernel_id": args.kernel, "precision": args.precision,
+                "copy_bytes": COPY_BYTES, "lookup_ms": min(ms)}
+        if method == "event":
+            meta["lookups"] = args.lookups
+            meta["balg_bytes_per_lookup"] = dev.balg(args.lookups, precision=prec)
+        with open(args.meta, "w") as f:
+            json.dump(meta, f)
     dev.close()
 
 
