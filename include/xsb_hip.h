@@ -139,7 +139,8 @@ typedef struct {
 	int32_t tuned;     /* 1 if the autotuner ran during this call */
 	int32_t graph;     /* 1 if the pipeline ran as a hipGraph */
 	int32_t mapping;   /* 1: one lane per lookup; 2: wave-uniform path where a wave's
-	                      lanes share the material (unionized FP64) */
+	                      lanes share the material (unionized grid); 3: as 2 without
+	                      the three-point shared step (unionized FP64) */
 	int32_t reserved0;
 	double ms_tuning;  /* host wall time spent tuning during this call */
 } xsb_batch_plan;
@@ -158,7 +159,8 @@ typedef struct {
 	int32_t sort;              /* mode 0: 1 or 2 (0 = autotuned) */
 	int64_t window;            /* mode 0: lookups per sort window (0 = one window) */
 	int32_t eager;             /* mode 0: 1 = launch the kernels directly instead of the graph */
-	int32_t mapping;           /* mode 0: 1 lane per lookup, 2 wave-uniform (0 = autotuned) */
+	int32_t mapping;           /* mode 0: 1 lane per lookup, 2 wave-uniform, 3 wave-uniform
+	                              without the three-point step (0 = autotuned) */
 	uint64_t *out_vsum;        /* optional host [n_batches] */
 	double *out_done_ms;       /* optional host [n_batches]: completion time of each batch,
 	                              from the start of the call's first kernel */
@@ -185,7 +187,8 @@ typedef struct {
 	int64_t bytes_fp64;       /* FP64 AoS grid */
 	int64_t bytes_fp32;       /* FP32 XS table (0 if not built) */
 	int64_t bytes_bf16;       /* BF16 XS table (0 if not built) */
-	int64_t bytes_index;      /* index grid */
+	int64_t bytes_index;      /* index grid (rows padded to 16 bytes) */
+	int64_t bytes_midx;       /* per-material copies of the unionized index grid */
 	int64_t bytes_union;      /* unionized energies */
 	int32_t cu_count;
 	int32_t xcd_count;

@@ -112,7 +112,7 @@ static void release(Device *d)
 	batch_state_free(d);
 	if (d->history_graph.exec)
 		(void)hipGraphExecDestroy(d->history_graph.exec);
-	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_grid,  d->d_egrid,    d->d_xs32, d->d_xs16,
+	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_midx, d->d_grid,  d->d_egrid,    d->d_xs32, d->d_xs16,
 	                d->d_xs16_scale, d->d_num_nucs, d->d_mats, d->d_concs, d->d_concs32, d->d_acc,
 	                d->d_fftab};
 	for (void *p : ptrs)
@@ -415,6 +415,52 @@ static int download(Device *d, double *grid, double *ueg, int32_t *index)
 	return XSB_OK;
 }
 
+// Material m's unionized index table: row e holds index_grid[e][mats[m][j]]
+// for j < nn, then zeros up to the pitch mp.
+__global__ void k_material_index(const int32_t *__restrict__ index, int64_t rows, int32_t pitch,
+                                 const int32_t *__restrict__ ml, int32_t nn, int32_t mp,
+                                 int32_t *__restrict__ dst)
+{
+	const int64_t total = rows * mp;
+	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+	     t += (int64_t)gridDim.x * blockDim.x) {
+		const int64_t e = t / mp;
+		const int j = (int)(t - e * mp);
+		dst[t] = (j < nn) ? index[e * pitch + ml[j]] : 0;
+	}
+}
+
+// Per-material copies of the unionized index grid (Tables::midx): each
+// lookup then reads its material's brackets from one contiguous row.
+static int build_material_index(Device *d, const xsb_tables *t, int64_t rows, int64_t pitch)
+{
+	Tables &T = d->T;
+	int64_t total = 0;
+	for (int m = 0; m < kMaterials; m++) {
+		T.midx_off[m] = total;
+		T.midx_pitch[m] = (t->num_nucs[m] + 3) & ~3;
+		total += rows * T.midx_pitch[m];
+	}
+	const size_t bytes = (size_t)total * sizeof(int32_t);
+	if (hipMalloc((void **)&d->d_midx, bytes) != hipSuccess) {
+		(void)hipGetLastError();
+		d->d_midx = nullptr;
+		return fail(XSB_ERR_OOM, "hipMalloc of %zu bytes (material index tables) failed", bytes);
+	}
+	d->bytes_midx = (int64_t)bytes;
+	for (int m = 0; m < kMaterials; m++) {
+		if (!T.midx_pitch[m])
+			continue;
+		hipLaunchKernelGGL(k_material_index, dim3(4096), dim3(256), 0, d->stream, d->d_index, rows,
+		                   (int32_t)pitch, d->d_mats + (int64_t)m * t->max_num_nucs,
+		                   t->num_nucs[m], T.midx_pitch[m], d->d_midx + T.midx_off[m]);
+		XSB_HIP(hipGetLastError());
+	}
+	XSB_HIP(hipStreamSynchronize(d->stream));
+	T.midx = d->d_midx;
+	return XSB_OK;
+}
+
 static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 {
 	int rc = validate_tables(t, flags);
@@ -527,27 +573,8 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	T.n_iso = (int32_t)n_iso;
 	T.idx_pitch = (int32_t)pitch;
 	T.variant = kVariantDefault;
-	// per material: the longest run of consecutive ids that starts at a
-	// multiple of 4, cut to a multiple of 4 (read as 16-byte vectors)
-	for (int m = 0; m < kMaterials; m++) {
-		const int32_t *ml = t->mats + m * t->max_num_nucs;
-		const int nn = t->num_nucs[m];
-		int best_s = 0, best_l = 0;
-		for (int j = 0; j < nn; j++) {
-			if (ml[j] % 4 != 0)
-				continue;
-			int l = 1;
-			while (j + l < nn && ml[j + l] == ml[j] + l)
-				l++;
-			l &= ~3;
-			if (l > best_l) {
-				best_l = l;
-				best_s = j;
-			}
-		}
-		T.vec_start[m] = best_s;
-		T.vec_len[m] = best_l;
-	}
+	if (t->grid_type == XSB_UNIONIZED && (rc = build_material_index(d, t, n_union, pitch)))
+		return rc;
 	T.n_gp = (int32_t)n_gp;
 	T.hash_bins = t->hash_bins;
 	T.max_num_nucs = t->max_num_nucs;
@@ -903,6 +930,7 @@ int xsb_hip_info(void *handle, xsb_info *info)
 	info->bytes_fp32 = d->bytes_fp32;
 	info->bytes_bf16 = d->bytes_bf16;
 	info->bytes_index = d->bytes_index;
+	info->bytes_midx = d->bytes_midx;
 	info->bytes_union = d->bytes_union;
 	info->cu_count = d->cu_count;
 	info->xcd_count = d->xcd_count;

@@ -347,7 +347,7 @@ __device__ __forceinline__ void flush_batches(const BatchView &B, const SortedQu
 	}
 }
 
-template <int GRID, int PREC, int CHUNK>
+template <int GRID, int PREC, int CHUNK, bool SHARE3>
 __global__ __launch_bounds__(kBatchBlock) void k_bsorted(Tables T, BatchView B, SortedQueue q)
 {
 	__shared__ uint32_t s_off[kLdsBatches + 1];
@@ -403,8 +403,8 @@ __global__ __launch_bounds__(kBatchBlock) void k_bsorted(Tables T, BatchView B, 
 			int64_t idx;
 			double macro[5];
 			uint32_t fb = 0;
-			const int am = sorted_lookup<GRID, PREC, CHUNK>(T, p, mat, q.wave, idx, macro, fb,
-			                                                nullptr);
+			const int am = sorted_lookup<GRID, PREC, CHUNK, false, SHARE3>(T, p, mat, q.wave, idx,
+			                                                               macro, fb, nullptr);
 			if (valid) {
 				fb_total += fb;
 				if (lds) {
@@ -433,69 +433,73 @@ __global__ __launch_bounds__(kBatchBlock) void k_bsorted(Tables T, BatchView B, 
 		block_add_u64<kBatchBlock>(fb_total, q.nfb);
 }
 
-template <int GRID, int PREC, int CHUNK>
+template <int GRID, int PREC, int CHUNK, bool SHARE3>
 static int bsorted_occupancy(int *blocks)
 {
-	XSB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k_bsorted<GRID, PREC, CHUNK>,
-	                                                     kBatchBlock, 0));
+	XSB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+	    blocks, k_bsorted<GRID, PREC, CHUNK, SHARE3>, kBatchBlock, 0));
 	return XSB_OK;
 }
 
-template <int GRID, int PREC, int CHUNK>
-static void launch_bsorted(Device *d, const BatchView &B, const SortedQueue &q, unsigned grid,
-                           hipStream_t s)
+template <int GRID, int PREC, int CHUNK, bool SHARE3>
+static int launch_or_size(Device *d, const BatchView &B, const SortedQueue &q, bool occupancy_only,
+                          int *occ, hipStream_t s)
 {
-	hipLaunchKernelGGL((k_bsorted<GRID, PREC, CHUNK>), dim3(grid), dim3(kBatchBlock), 0, s, d->T, B,
-	                   q);
-}
-
-// dispatch on (grid type, precision, staging depth); the depth only matters
-// for the unionized FP64 wave path, the others use one instantiation
-template <int GRID, int PREC>
-static int dispatch_bsorted(Device *d, const BatchView &B, const SortedQueue &q, int stage,
-                            bool occupancy_only, int *occ, hipStream_t s)
-{
-	const bool deep = !(PREC == XSB_XS_FP64 && GRID == kUnionized) || stage >= 16;
 	if (occupancy_only)
-		return deep ? bsorted_occupancy<GRID, PREC, 16>(occ) : bsorted_occupancy<GRID, PREC, 8>(occ);
-	const unsigned grid = (unsigned)*occ;
-	if (deep)
-		launch_bsorted<GRID, PREC, 16>(d, B, q, grid, s);
-	else
-		launch_bsorted<GRID, PREC, 8>(d, B, q, grid, s);
+		return bsorted_occupancy<GRID, PREC, CHUNK, SHARE3>(occ);
+	hipLaunchKernelGGL((k_bsorted<GRID, PREC, CHUNK, SHARE3>), dim3((unsigned)*occ),
+	                   dim3(kBatchBlock), 0, s, d->T, B, q);
 	XSB_HIP(hipGetLastError());
 	return XSB_OK;
 }
 
+// dispatch on (grid type, precision, staging depth, three-point step); the
+// last two only matter for the unionized FP64 wave path, the others use one
+// instantiation
+template <int GRID, int PREC>
+static int dispatch_bsorted(Device *d, const BatchView &B, const SortedQueue &q, int stage,
+                            bool share3, bool occupancy_only, int *occ, hipStream_t s)
+{
+	if (!(PREC == XSB_XS_FP64 && GRID == kUnionized))
+		return launch_or_size<GRID, PREC, 16, true>(d, B, q, occupancy_only, occ, s);
+	if (stage >= 16)
+		return share3 ? launch_or_size<GRID, PREC, 16, true>(d, B, q, occupancy_only, occ, s)
+		              : launch_or_size<GRID, PREC, 16, false>(d, B, q, occupancy_only, occ, s);
+	return share3 ? launch_or_size<GRID, PREC, 8, true>(d, B, q, occupancy_only, occ, s)
+	              : launch_or_size<GRID, PREC, 8, false>(d, B, q, occupancy_only, occ, s);
+}
+
 template <int GRID>
 static int dispatch_bsorted_prec(Device *d, int prec, const BatchView &B, const SortedQueue &q,
-                                 int stage, bool occupancy_only, int *occ, hipStream_t s)
+                                 int stage, bool share3, bool occupancy_only, int *occ,
+                                 hipStream_t s)
 {
 	if (prec == XSB_XS_FP64)
-		return dispatch_bsorted<GRID, XSB_XS_FP64>(d, B, q, stage, occupancy_only, occ, s);
+		return dispatch_bsorted<GRID, XSB_XS_FP64>(d, B, q, stage, share3, occupancy_only, occ, s);
 	if (prec == XSB_XS_FP32)
-		return dispatch_bsorted<GRID, XSB_XS_FP32>(d, B, q, stage, occupancy_only, occ, s);
-	return dispatch_bsorted<GRID, XSB_XS_BF16>(d, B, q, stage, occupancy_only, occ, s);
+		return dispatch_bsorted<GRID, XSB_XS_FP32>(d, B, q, stage, share3, occupancy_only, occ, s);
+	return dispatch_bsorted<GRID, XSB_XS_BF16>(d, B, q, stage, share3, occupancy_only, occ, s);
 }
 
 // occupancy_only: *occ receives resident workgroups per CU; else *occ is
 // the grid size to launch
 static int run_bsorted(Device *d, int prec, const BatchView &B, const SortedQueue &q, int stage,
-                       bool occupancy_only, int *occ, hipStream_t s)
+                       bool share3, bool occupancy_only, int *occ, hipStream_t s)
 {
 	switch (d->T.grid_type) {
 	case kUnionized:
-		return dispatch_bsorted_prec<kUnionized>(d, prec, B, q, stage, occupancy_only, occ, s);
+		return dispatch_bsorted_prec<kUnionized>(d, prec, B, q, stage, share3, occupancy_only, occ, s);
 	case kNuclide:
-		return dispatch_bsorted_prec<kNuclide>(d, prec, B, q, stage, occupancy_only, occ, s);
+		return dispatch_bsorted_prec<kNuclide>(d, prec, B, q, stage, share3, occupancy_only, occ, s);
 	default:
-		return dispatch_bsorted_prec<kHash>(d, prec, B, q, stage, occupancy_only, occ, s);
+		return dispatch_bsorted_prec<kHash>(d, prec, B, q, stage, share3, occupancy_only, occ, s);
 	}
 }
 
 // ---- host side of mode 0 ----
 // mapping: 1 one lane per lookup (the reference loop per lane), 2 the
-// wave-uniform path where a wave's lanes share the material
+// wave-uniform path where a wave's lanes share the material, 3 the same
+// without the three-point shared step (unionized FP64)
 struct BatchPlan {
 	int tile = 0, stage = 0, sort = 0, mapping = 0;
 	bool fits(const BatchPlan &fixed) const
@@ -606,9 +610,9 @@ static int enqueue_pipeline(Device *d, const PipeShape &ps, const BatchView &B, 
 	q.n_items = (ps.n + ps.plan.tile - 1) / ps.plan.tile;
 	q.ebits = ps.ebits;
 	q.flush_each_item = ps.windows > 1;
-	q.wave = ps.plan.mapping == 2;
+	q.wave = ps.plan.mapping >= 2;
 	int grid = ps.grid;
-	if ((rc = run_bsorted(d, ps.prec, B, q, ps.plan.stage, false, &grid, s)))
+	if ((rc = run_bsorted(d, ps.prec, B, q, ps.plan.stage, ps.plan.mapping == 2, false, &grid, s)))
 		return rc;
 	hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, clk + 2);
 	XSB_HIP(hipGetLastError());
@@ -632,7 +636,8 @@ static int make_shape(Device *d, int nb, int64_t n, int windows, int prec, const
 	const int bits = wbits + 4 + ps->ebits;
 	ps->passes = (bits + kDigitBits - 1) / kDigitBits;
 	int occ = 0;
-	int rc = run_bsorted(d, prec, BatchView{}, SortedQueue{}, plan.stage, true, &occ, nullptr);
+	int rc = run_bsorted(d, prec, BatchView{}, SortedQueue{}, plan.stage, plan.mapping == 2, true,
+	                     &occ, nullptr);
 	if (rc)
 		return rc;
 	ps->grid = std::max(1, d->cu_count) * std::max(1, occ);
@@ -703,14 +708,17 @@ static int autotune(Device *d, const xsb_batched_args *a, const std::vector<uint
 	const bool staged = d->T.grid_type == kUnionized && a->xs_precision == XSB_XS_FP64;
 	const std::vector<int> tiles = pick(fixed.tile, {2048, 8192});
 	const std::vector<int> sorts = pick(fixed.sort, {2, 1});
-	const std::vector<int> maps = staged ? pick(fixed.mapping, {2, 1}) : pick(fixed.mapping, {1});
+	const std::vector<int> maps =
+	    staged ? pick(fixed.mapping, {2, 3, 1})
+	           : pick(fixed.mapping, d->T.grid_type == kUnionized ? std::vector<int>{2, 1}
+	                                                              : std::vector<int>{1});
 	float best = 1e30f;
 	BatchPlan best_plan;
 	for (int so : sorts)
 		for (int ti : tiles)
 			for (int mp : maps) {
 				const std::vector<int> stages =
-				    (staged && mp == 2) ? pick(fixed.stage, {16, 8}) : pick(fixed.stage, {16});
+				    (staged && mp >= 2) ? pick(fixed.stage, {16, 8}) : pick(fixed.stage, {16});
 				for (int st : stages) {
 					const BatchPlan c{ti, st, so, mp};
 					float ms = 0, ms2 = 0;
@@ -783,7 +791,7 @@ static int run_sorted_batches(Device *d, const xsb_batched_args *a, xsb_result *
 	fixed.tile = a->tile > 0 ? std::max(kBatchBlock, (a->tile + kBatchBlock - 1) / kBatchBlock * kBatchBlock) : 0;
 	fixed.stage = a->stage > 0 ? (a->stage >= 16 ? 16 : 8) : 0;
 	fixed.sort = a->sort > 0 ? (a->sort >= 2 ? 2 : 1) : 0;
-	fixed.mapping = a->mapping > 0 ? (a->mapping >= 2 ? 2 : 1) : 0;
+	fixed.mapping = a->mapping > 0 ? std::min(a->mapping, 3) : 0;
 	BatchPlan plan = fixed;
 	int tuned = 0;
 	double ms_tuning = 0;

@@ -45,11 +45,12 @@ struct Tables {
 	int32_t variant;  // kVariant* bits: lookup-path choices (tuning, same results)
 	double thr[kMaterials];  // pick_mat thresholds, reverse-order partial sums
 	double hash_du;          // 1.0 / hash_bins, as computed by the reference
-	// per material: positions [vec_start, vec_start + vec_len) of its list
-	// hold consecutive nuclide ids starting at a multiple of 4, vec_len a
-	// multiple of 4 (the large fuel's ids 68..) -- read as 16-byte vectors
-	int32_t vec_start[kMaterials];
-	int32_t vec_len[kMaterials];
+	// unionized grid, per material m: its own copy of the index grid holding
+	// only its nuclides' columns, in material-list order, rows padded to a
+	// multiple of 4 ints -- row e at midx + midx_off[m] + e * midx_pitch[m]
+	const int32_t *__restrict__ midx;
+	int64_t midx_off[kMaterials];
+	int32_t midx_pitch[kMaterials];
 };
 
 // ---------------------------------------------------------------------------
@@ -176,14 +177,24 @@ __device__ __forceinline__ int32_t grid_search_nuclide(double q, const double *_
 // point index `lower` such that points lower and lower+1 are interpolated.
 // E_row points at the nuclide's first energy word, stored with stride STRIDE.
 // ---------------------------------------------------------------------------
+// Unionized grid: row idx of material m's index table (entry j = the
+// bracket of the material's j-th nuclide).
+__device__ __forceinline__ const int32_t *ueg_row(const Tables &T, int m, int64_t idx)
+{
+	return T.midx + T.midx_off[m] + idx * T.midx_pitch[m];
+}
+
+// mrow: the material's unionized index row (ueg_row), j: the nuclide's
+// position in the material list; both used on the unionized grid only.
 template <int GRID, int STRIDE>
 __device__ __forceinline__ int32_t nuclide_lower(const Tables &T, const double *__restrict__ E_row,
-                                                 int nuc, int64_t idx, double p)
+                                                 int nuc, const int32_t *__restrict__ mrow, int j,
+                                                 int64_t idx, double p)
 {
 	const int32_t n_gp = T.n_gp;
 	int32_t lower;
 	if (GRID == kUnionized) {
-		lower = T.index_grid[idx * T.idx_pitch + nuc];
+		lower = mrow[j];
 	} else if (GRID == kNuclide) {
 		lower = grid_search_nuclide<STRIDE>(p, E_row, 0, n_gp - 1);
 	} else {
@@ -250,11 +261,12 @@ __device__ __forceinline__ void macro_xs_fp64(const Tables &T, double p, int mat
 	const int nn = T.num_nucs[mat];
 	const int32_t *__restrict__ ml = T.mats + mat * T.max_num_nucs;
 	const double *__restrict__ cl = T.concs + mat * T.max_num_nucs;
+	const int32_t *__restrict__ mrow = (GRID == kUnionized) ? ueg_row(T, mat, idx) : nullptr;
 	for (int j = 0; j < nn; j++) {
 		const int nuc = ml[j];
 		const double conc = cl[j];
 		const double *row = T.grid + (int64_t)nuc * T.n_gp * kPointDoubles;
-		const int32_t lower = nuclide_lower<GRID, kPointDoubles>(T, row, nuc, idx, p);
+		const int32_t lower = nuclide_lower<GRID, kPointDoubles>(T, row, nuc, mrow, j, idx, p);
 		accumulate_fp64(row + (int64_t)lower * kPointDoubles, p, conc, macro);
 	}
 }
@@ -282,8 +294,12 @@ __device__ __forceinline__ void accumulate_fp64_select(const double *__restrict_
 
 // One nuclide of the wave-uniform unionized path, bracket k per lane.  The
 // points come through the scalar cache when the lanes' brackets coincide
-// (one pair for the wave) or span two neighbouring intervals (three points,
-// each lane picks its pair), else per lane.
+// (one pair for the wave), and with SHARE3 also when they span two
+// neighbouring intervals (three points, each lane picks its pair); else per
+// lane.  The scheduling barrier after a three-point step keeps the compiler
+// from overlapping several of them (which held ~200 VGPRs live across the
+// unrolled chunk: two waves per SIMD instead of six).
+template <bool SHARE3>
 __device__ __forceinline__ void ueg_wave_nuclide(const Tables &T, int nuc, int k, double p,
                                                  double conc, double macro[5])
 {
@@ -293,76 +309,64 @@ __device__ __forceinline__ void ueg_wave_nuclide(const Tables &T, int nuc, int k
 	const int k0 = __builtin_amdgcn_readfirstlane(k);
 	const int d = k - k0;
 	const double *P = T.grid + ((int64_t)nuc * n_gp + k0) * kPointDoubles;
-	if (__ballot(d != 0) == 0ULL)
+	if (__ballot(d != 0) == 0ULL) {
 		accumulate_fp64(P, p, conc, macro);
-	else if (__ballot(d != 0 && d != 1) == 0ULL)
+	} else if (SHARE3 && __ballot(d != 0 && d != 1) == 0ULL) {
 		accumulate_fp64_select(P, d, p, conc, macro);
-	else if (__ballot(d != 0 && d != -1) == 0ULL)
+		__builtin_amdgcn_sched_barrier(0);
+	} else if (SHARE3 && __ballot(d != 0 && d != -1) == 0ULL) {
 		accumulate_fp64_select(P - kPointDoubles, d + 1, p, conc, macro);
-	else
+		__builtin_amdgcn_sched_barrier(0);
+	} else {
 		accumulate_fp64(T.grid + ((int64_t)nuc * n_gp + k) * kPointDoubles, p, conc, macro);
+	}
 }
 
 // Walks material m's nuclides in list order for the wave-uniform unionized
-// paths, calling f(j, nuclide, bracket k of this lane) for each.  A lane's
-// index-grid words are fetched CHUNK (the staging depth, a multiple of 4) at
-// a time, all in flight together, so its row is read once per chunk rather
-// than once per nuclide; over the material's run of consecutive ids (the
-// large fuel's 68..354, 16-byte aligned in the padded rows) they are fetched
-// as 16-byte vectors.
+// paths, calling f(j, nuclide, bracket k of this lane) for each.  The lane's
+// brackets are contiguous in its row of the material's index table, so they
+// are fetched as 16-byte vectors, CHUNK (the staging depth, a multiple of 4)
+// at a time, all in flight together.
 template <int CHUNK, class F>
 __device__ __forceinline__ void ueg_wave_rows(const Tables &T, int m, int64_t idx, F &&f)
 {
+	static_assert(CHUNK % 4 == 0, "staging depth is whole 16-byte vectors");
 	const int nn = T.num_nucs[m];
 	const int32_t *__restrict__ ml = T.mats + m * T.max_num_nucs;
-	const int32_t *__restrict__ row = T.index_grid + idx * T.idx_pitch;
-	const int vs = T.vec_start[m], ve = vs + T.vec_len[m];
-	auto scattered = [&](int j0, int j1) {
-		for (; j0 + CHUNK <= j1; j0 += CHUNK) {
-			int kk[CHUNK];
-#pragma unroll
-			for (int u = 0; u < CHUNK; u++)
-				kk[u] = row[ml[j0 + u]];
-#pragma unroll
-			for (int u = 0; u < CHUNK; u++)
-				f(j0 + u, ml[j0 + u], kk[u]);
-		}
-		for (; j0 < j1; j0++)
-			f(j0, ml[j0], row[ml[j0]]);
-	};
-	scattered(0, vs);
-	int j0 = vs;
-	for (; j0 + CHUNK <= ve; j0 += CHUNK) {
-		const int4 *q = reinterpret_cast<const int4 *>(row + ml[j0]);
+	const int4 *__restrict__ row = reinterpret_cast<const int4 *>(ueg_row(T, m, idx));
+	int j0 = 0;
+	for (; j0 + CHUNK <= nn; j0 += CHUNK) {
 		int4 kk[CHUNK / 4];
 #pragma unroll
 		for (int u = 0; u < CHUNK / 4; u++)
-			kk[u] = q[u];
+			kk[u] = row[j0 / 4 + u];
 #pragma unroll
 		for (int u = 0; u < CHUNK / 4; u++) {
-			const int j = j0 + 4 * u, nuc = ml[j];
-			f(j, nuc, kk[u].x);
-			f(j + 1, nuc + 1, kk[u].y);
-			f(j + 2, nuc + 2, kk[u].z);
-			f(j + 3, nuc + 3, kk[u].w);
+			const int j = j0 + 4 * u;
+			f(j, ml[j], kk[u].x);
+			f(j + 1, ml[j + 1], kk[u].y);
+			f(j + 2, ml[j + 2], kk[u].z);
+			f(j + 3, ml[j + 3], kk[u].w);
 		}
 	}
-	for (; j0 < ve; j0 += 4) {
-		const int4 k4 = *reinterpret_cast<const int4 *>(row + ml[j0]);
-		const int nuc = ml[j0];
-		f(j0, nuc, k4.x);
-		f(j0 + 1, nuc + 1, k4.y);
-		f(j0 + 2, nuc + 2, k4.z);
-		f(j0 + 3, nuc + 3, k4.w);
+	// rows are padded to a multiple of 4, so the last vector is in bounds
+	for (; j0 < nn; j0 += 4) {
+		const int4 k4 = row[j0 / 4];
+		f(j0, ml[j0], k4.x);
+		if (j0 + 1 < nn)
+			f(j0 + 1, ml[j0 + 1], k4.y);
+		if (j0 + 2 < nn)
+			f(j0 + 2, ml[j0 + 2], k4.z);
+		if (j0 + 3 < nn)
+			f(j0 + 3, ml[j0 + 3], k4.w);
 	}
-	scattered(ve, nn);
 }
 
 // calculate_macro_xs, unionized grid, FP64, for a wave whose lanes all hold
 // material m (the material-sorted schedules).  The material list, the
 // concentrations and the shared grid points are wave-uniform (scalar
 // cache).  Arithmetic and summation order are those of macro_xs_fp64.
-template <int CHUNK>
+template <int CHUNK, bool SHARE3 = true>
 __device__ __forceinline__ void macro_ueg_wave(const Tables &T, double p, int m, int64_t idx,
                                                double macro[5])
 {
@@ -371,7 +375,7 @@ __device__ __forceinline__ void macro_ueg_wave(const Tables &T, double p, int m,
 		macro[c] = 0.0;
 	const double *__restrict__ cl = T.concs + m * T.max_num_nucs;
 	ueg_wave_rows<CHUNK>(T, m, idx, [&](int j, int nuc, int k) {
-		ueg_wave_nuclide(T, nuc, k, p, cl[j], macro);
+		ueg_wave_nuclide<SHARE3>(T, nuc, k, p, cl[j], macro);
 	});
 }
 

@@ -171,13 +171,14 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 	const float *__restrict__ ceff = mixed_consts<PREC>(T, mat);
 	const float *__restrict__ cbnd = ceff + T.max_num_nucs * kMaterials;
 	const int32_t *__restrict__ ml = T.mats + mat * T.max_num_nucs;
+	const int32_t *__restrict__ mrow = (GRID == kUnionized) ? ueg_row(T, mat, idx) : nullptr;
 	float sum[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
 	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
 	float A = 0.f;
 	for (int j = 0; j < nn; j++) {
 		const int nuc = ml[j];
 		const double *row = mixed_row<PREC>(T, nuc);
-		const int32_t lower = nuclide_lower<GRID, REC>(T, row, nuc, idx, p);
+		const int32_t lower = nuclide_lower<GRID, REC>(T, row, nuc, mrow, j, idx, p);
 		const double *lo = row + (int64_t)lower * REC;
 		const double *hi = lo + REC;
 		float xl[5], xh[5];

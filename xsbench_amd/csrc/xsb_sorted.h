@@ -53,11 +53,12 @@ __device__ __forceinline__ void write_tile_hist(const uint32_t *h, uint32_t *his
 // One lane's lookup (p, mat).  Unionized waves whose lanes share the
 // material take the wave-uniform path (macro_ueg_wave / _mixed, index words
 // staged CHUNK deep) when `wave` is set; everything else runs the per-lane
-// loop.  Returns the argmax channel; idx gets the grid-level index,
+// loop.  SHARE3 selects the FP64 wave path's three-point step
+// (ueg_wave_nuclide).  Returns the argmax channel; idx gets the grid-level index,
 // macro the sums, fb the fallback flag (mixed precision).
 // HISTORY: the caller also tests macro_c > 1.0, so mixed precision falls
 // back to FP64 near 1.0 too and then returns the FP64 sums.
-template <int GRID, int PREC, int CHUNK, bool HISTORY = false>
+template <int GRID, int PREC, int CHUNK, bool HISTORY = false, bool SHARE3 = true>
 __device__ __forceinline__ int sorted_lookup(const Tables &T, double p, int mat, bool wave,
                                              int64_t &idx, double macro[5], uint32_t &fb,
                                              double *bnd)
@@ -66,7 +67,7 @@ __device__ __forceinline__ int sorted_lookup(const Tables &T, double p, int mat,
 	if (PREC == XSB_XS_FP64 && GRID == kUnionized) {
 		const int m0 = __builtin_amdgcn_readfirstlane(mat);
 		if (wave && __ballot(mat != m0) == 0ULL)
-			macro_ueg_wave<CHUNK>(T, p, m0, idx, macro);
+			macro_ueg_wave<CHUNK, SHARE3>(T, p, m0, idx, macro);
 		else
 			macro_xs_fp64<GRID>(T, p, mat, idx, macro);
 		return argmax5(macro);
