@@ -598,7 +598,7 @@ static int enqueue_pipeline(Device *d, const PipeShape &ps, const BatchView &B, 
 	hipLaunchKernelGGL(k_bkeys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, B, ps.n, ps.ebits, kA, vA,
 	                   hist, ntiles);
 	XSB_HIP(hipGetLastError());
-	int rc = sort_pairs(s, ps.n, ps.passes, true, &kA, &vA, &kB, &vB, hist, part);
+	int rc = order_pairs(s, ps.n, ps.passes, true, &kA, &vA, &kB, &vB, hist, part);
 	if (rc)
 		return rc;
 	hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, clk + 1);

@@ -167,7 +167,7 @@ static int enqueue_rounds(Device *d, hipStream_t s, int64_t P, int lookups, uint
 		hipLaunchKernelGGL(k_hkeys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, P, B.seed, kA, vA,
 		                   B.hist, ntiles);
 		XSB_HIP(hipGetLastError());
-		int rc = sort_pairs(s, P, 2, true, &kA, &vA, &kB, &vB, B.hist, B.part);
+		int rc = order_pairs(s, P, 2, true, &kA, &vA, &kB, &vB, B.hist, B.part);
 		if (rc)
 			return rc;
 		if (prec == XSB_XS_FP64)

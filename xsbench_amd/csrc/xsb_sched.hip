@@ -235,6 +235,74 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ ki
 	}
 }
 
+// Scatter of one digit for the lookup schedules, which need the order only
+// for locality: a tile's elements are grouped by digit in LDS (ranks from
+// LDS counters, so equal digits of one tile may swap places), then written
+// out in runs, consecutive lanes storing consecutive addresses.  After the
+// last pass the keys are sorted; elements with equal keys may be permuted
+// within a tile.
+__global__ __launch_bounds__(kSortBlock) void k_scatter_runs(const uint32_t *__restrict__ kin,
+                                                             const uint32_t *__restrict__ vin,
+                                                             uint32_t *__restrict__ kout,
+                                                             uint32_t *__restrict__ vout,
+                                                             int64_t n, int shift,
+                                                             const uint32_t *__restrict__ hist,
+                                                             int ntiles)
+{
+	__shared__ uint32_t cnt[kBins];   // digit counts, then the tile's local starts
+	__shared__ uint32_t gbase[kBins]; // global start minus local start, per digit
+	__shared__ uint32_t sk[kTile], sv[kTile];
+	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
+		cnt[d] = 0;
+	__syncthreads();
+	const int64_t base = (int64_t)blockIdx.x * kTile;
+	const int m = (int)min((int64_t)kTile, n - base);
+	uint32_t kk[kSortItems], vv[kSortItems], rank[kSortItems];
+#pragma unroll
+	for (int it = 0; it < kSortItems; it++) {
+		const int i = it * kSortBlock + threadIdx.x;
+		if (i < m) {
+			kk[it] = kin[base + i];
+			vv[it] = vin[base + i];
+			rank[it] = atomicAdd(&cnt[(kk[it] >> shift) & kDigitMask], 1u);
+		}
+	}
+	__syncthreads();
+	// local exclusive scan: thread t owns digits 4t .. 4t+3
+	static_assert(kBins == 4 * kSortBlock, "four digits per thread");
+	uint32_t c[4], sum = 0;
+#pragma unroll
+	for (int j = 0; j < 4; j++) {
+		c[j] = cnt[4 * threadIdx.x + j];
+		sum += c[j];
+	}
+	uint32_t run = block_excl_scan(sum, nullptr);
+#pragma unroll
+	for (int j = 0; j < 4; j++) {
+		const int d = 4 * threadIdx.x + j;
+		cnt[d] = run;
+		gbase[d] = hist[(int64_t)d * ntiles + blockIdx.x] - run;
+		run += c[j];
+	}
+	__syncthreads();
+#pragma unroll
+	for (int it = 0; it < kSortItems; it++) {
+		const int i = it * kSortBlock + threadIdx.x;
+		if (i < m) {
+			const uint32_t pos = cnt[(kk[it] >> shift) & kDigitMask] + rank[it];
+			sk[pos] = kk[it];
+			sv[pos] = vv[it];
+		}
+	}
+	__syncthreads();
+	for (int i = threadIdx.x; i < m; i += kSortBlock) {
+		const uint32_t key = sk[i];
+		const uint32_t g = gbase[(key >> shift) & kDigitMask] + (uint32_t)i;
+		kout[g] = key;
+		vout[g] = sv[i];
+	}
+}
+
 // first element of each digit after the final pass (for material counts)
 __global__ void k_digit_starts(const uint32_t *__restrict__ hist, int ntiles, uint32_t *out)
 {
@@ -354,6 +422,34 @@ int sort_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint32_t 
 	return sort_pairs_t<uint32_t>(s, n, passes, hist0_ready, k, v, k2, v2, hist, part);
 }
 
+int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint32_t **k, uint32_t **v,
+                uint32_t **k2, uint32_t **v2, uint32_t *hist, uint32_t *part)
+{
+	const int ntiles = sort_tiles(n);
+	const int64_t hlen = sort_hist_len(n);
+	int rc;
+	for (int pass = 0; pass < passes; pass++) {
+		const int shift = kDigitBits * pass;
+		if (pass > 0 || !hist0_ready) {
+			hipLaunchKernelGGL(k_hist<uint32_t>, dim3(ntiles), dim3(kSortBlock), 0, s, *k, n, shift,
+			                   hist, ntiles);
+			XSB_HIP(hipGetLastError());
+		}
+		if ((rc = scan_hist(hist, hlen, part, s)))
+			return rc;
+		hipLaunchKernelGGL(k_scatter_runs, dim3(ntiles), dim3(kSortBlock), 0, s, *k, *v, *k2, *v2,
+		                   n, shift, hist, ntiles);
+		XSB_HIP(hipGetLastError());
+		uint32_t *t = *k;
+		*k = *k2;
+		*k2 = t;
+		t = *v;
+		*v = *v2;
+		*v2 = t;
+	}
+	return XSB_OK;
+}
+
 int sort_pairs64(hipStream_t s, int64_t n, int passes, uint64_t **k, uint32_t **v, uint64_t **k2,
                  uint32_t **v2, uint32_t *hist, uint32_t *part)
 {
@@ -389,7 +485,7 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	hipLaunchKernelGGL(k_keys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, a->first_lookup, n,
 	                   a->seed, mode, kA, vA, hist, ntiles);
 	XSB_HIP(hipGetLastError());
-	if ((rc = sort_pairs(s, n, passes, true, &kA, &vA, &kB, &vB, hist, part)))
+	if ((rc = order_pairs(s, n, passes, true, &kA, &vA, &kB, &vB, hist, part)))
 		return rc;
 	hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(256), 0, s, hist, ntiles, starts);
 	XSB_HIP(hipGetLastError());
