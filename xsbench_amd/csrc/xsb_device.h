@@ -381,6 +381,60 @@ __device__ __forceinline__ void macro_ueg_wave(const Tables &T, double p, int m,
 
 constexpr int kIdxChunk = 16;
 
+// Hash grid: the bracket of one nuclide for a lookup in bin idx
+// (Simulation.c:273-302), with the bin's bounds u_low / u_high given.  When
+// every lane of the wave shares the bin, the bounds and their energies are
+// wave-uniform (scalar loads) and only the short bisection is per lane.
+template <int STRIDE>
+__device__ __forceinline__ int32_t hash_bracket(const double *__restrict__ E_row, int32_t u_low,
+                                                int32_t u_high, int32_t n_gp, double p)
+{
+	const double e_low = E_row[(int64_t)u_low * STRIDE];
+	const double e_high = E_row[(int64_t)u_high * STRIDE];
+	if (p <= e_low)
+		return 0;
+	if (p >= e_high)
+		return n_gp - 1;
+	return grid_search_nuclide<STRIDE>(p, E_row, u_low, u_high);
+}
+
+// Walks material m's nuclides for a wave whose lanes share the material and
+// the hash bin idx, calling f(j, nuclide, bracket k of this lane).
+template <int STRIDE, class F>
+__device__ __forceinline__ void hash_wave_rows(const Tables &T, const double *__restrict__ base,
+                                               double p, int m, int64_t idx, F &&f)
+{
+	const int nn = T.num_nucs[m];
+	const int32_t n_gp = T.n_gp;
+	const int32_t *__restrict__ ml = T.mats + m * T.max_num_nucs;
+	const int32_t *__restrict__ lo_row = T.index_grid + idx * T.idx_pitch;
+	const bool last = idx == T.hash_bins - 1;
+#pragma unroll 4
+	for (int j = 0; j < nn; j++) {
+		const int nuc = ml[j];
+		const int32_t u_low = lo_row[nuc];
+		const int32_t u_high = last ? n_gp - 1 : lo_row[T.idx_pitch + nuc] + 1;
+		const double *E_row = base + (int64_t)nuc * n_gp * STRIDE;
+		f(j, nuc, hash_bracket<STRIDE>(E_row, u_low, u_high, n_gp, p));
+	}
+}
+
+// calculate_macro_xs, hash grid, FP64, for a wave whose lanes share the
+// material and the hash bin.  Arithmetic and summation order are those of
+// macro_xs_fp64.
+template <bool SHARE3 = true>
+__device__ __forceinline__ void macro_hash_wave(const Tables &T, double p, int m, int64_t idx,
+                                                double macro[5])
+{
+#pragma unroll
+	for (int c = 0; c < 5; c++)
+		macro[c] = 0.0;
+	const double *__restrict__ cl = T.concs + m * T.max_num_nucs;
+	hash_wave_rows<kPointDoubles>(T, T.grid, p, m, idx, [&](int j, int nuc, int k) {
+		ueg_wave_nuclide<SHARE3>(T, nuc, k, p, cl[j], macro);
+	});
+}
+
 // argmax with max = -1.0 and strict '>' (Simulation.c:100-110)
 __device__ __forceinline__ int argmax5(const double m[5])
 {

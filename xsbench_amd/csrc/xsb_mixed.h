@@ -252,4 +252,23 @@ __device__ __forceinline__ int macro_ueg_wave_mixed(const Tables &T, double p, i
 	return mixed_finish<kUnionized, PREC, HISTORY>(T, p, m, idx, sum, A, macro, fell_back, bound);
 }
 
+// calculate_macro_xs, hash grid, mixed tables, for a wave whose lanes share
+// the material and the hash bin; same terms and decision as macro_xs_mixed.
+template <int PREC, bool HISTORY>
+__device__ __forceinline__ int macro_hash_wave_mixed(const Tables &T, double p, int m, int64_t idx,
+                                                     double macro[5], uint32_t *fell_back,
+                                                     double *bound)
+{
+	constexpr int REC = (PREC == XSB_XS_FP32) ? kRec32Doubles : kRec16Doubles;
+	const float *__restrict__ ceff = mixed_consts<PREC>(T, m);
+	const float *__restrict__ cbnd = ceff + T.max_num_nucs * kMaterials;
+	float sum[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+	float A = 0.f;
+	hash_wave_rows<REC>(T, mixed_row<PREC>(T, 0), p, m, idx, [&](int j, int nuc, int k) {
+		ueg_wave_nuclide_mixed<PREC>(T, nuc, k, p, ceff[j], cbnd[j], sum, comp, A);
+	});
+	return mixed_finish<kHash, PREC, HISTORY>(T, p, m, idx, sum, A, macro, fell_back, bound);
+}
+
 }  // namespace xsb

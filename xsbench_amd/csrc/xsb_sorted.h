@@ -77,13 +77,25 @@ __device__ __forceinline__ int sorted_lookup(const Tables &T, double p, int mat,
 		else
 			macro_xs_fp64<GRID>(T, p, mat, idx, macro);
 		return argmax5(macro);
-	} else if (PREC == XSB_XS_FP64) {
-		macro_xs_fp64<GRID>(T, p, mat, idx, macro);
-		return argmax5(macro);
-	} else if (GRID == kUnionized) {
+	} else if (GRID == kHash) {
+		// wave path when the lanes share the material and the hash bin
+		const int m0 = __builtin_amdgcn_readfirstlane(mat);
+		const int i0 = __builtin_amdgcn_readfirstlane((int)idx);
+		if (wave && __ballot(mat != m0 || (int)idx != i0) == 0ULL) {
+			if (PREC == XSB_XS_FP64) {
+				macro_hash_wave<SHARE3>(T, p, m0, i0, macro);
+				return argmax5(macro);
+			}
+			return macro_hash_wave_mixed<PREC, HISTORY>(T, p, m0, i0, macro, &fb, bnd);
+		}
+	} else if (GRID == kUnionized && PREC != XSB_XS_FP64) {
 		const int m0 = __builtin_amdgcn_readfirstlane(mat);
 		if (wave && __ballot(mat != m0) == 0ULL)
 			return macro_ueg_wave_mixed<PREC, CHUNK, HISTORY>(T, p, m0, idx, macro, &fb, bnd);
+	}
+	if (PREC == XSB_XS_FP64) {
+		macro_xs_fp64<GRID>(T, p, mat, idx, macro);
+		return argmax5(macro);
 	}
 	return macro_xs_mixed<GRID, PREC, HISTORY>(T, p, mat, idx, macro, &fb, bnd);
 }
