@@ -237,14 +237,24 @@ def main():
 
     variants = {f"fp64_k{kk}_lookups_per_s": round(LOOKUPS / (ms * 1e-3), 1) for kk, ms in sched_ms.items()}
     if not args.no_variants:
+        import numpy as np
+        _, ref = dev.event(1_000_000, kernel_id=6, detail=True)
         for name, prec in (("fp32", X.XS_FP32), ("bf16", X.XS_BF16)):
             run(6, prec)
             rr = [run(6, prec) for _ in range(3)]
             ms = min(r["ms_total"] for r in rr)
             variants[f"{name}_k6_lookups_per_s"] = round(LOOKUPS / (ms * 1e-3), 1)
             variants[f"{name}_k6_fp64_reevaluations"] = rr[0]["n_fallback"]
-            log(f"{name} k=6: {ms:.3f} ms, {rr[0]['n_fallback']} FP64 re-evaluations")
+            _, det = dev.event(1_000_000, kernel_id=6, precision=prec, detail=True)
+            rel = np.abs(det["macro"] - ref["macro"]) / np.abs(ref["macro"])
+            variants[f"{name}_rel_err_max"] = [float(f"{v:.3e}") for v in rel.max(axis=0)]
+            variants[f"{name}_rel_err_mean"] = [float(f"{v:.3e}") for v in rel.mean(axis=0)]
+            log(f"{name} k=6: {ms:.3f} ms, {rr[0]['n_fallback']} FP64 re-evaluations, relative "
+                f"error per channel max {variants[f'{name}_rel_err_max']}")
         variants.update(batched_variants(X, dev))
+        if "fp64_k0_lookups_per_s" in variants:
+            variants["mixed_batched_fp32_vs_plain_fp64_k0"] = round(
+                variants["batched_fp32_lookups_per_s"] / variants["fp64_k0_lookups_per_s"], 2)
         variants.update(hash_variants(X))
         variants.update(history_variants(X))
 

@@ -222,6 +222,8 @@ int xsb_hip_history(void *handle, const xsb_history_args *args, xsb_result *res)
 /* openmp-threading/Simulation.c:326 calculate_macro_xs over n explicit
  * samples held in device memory: d_energy[n], d_mat[n] -> d_macro[n x 5]
  * (nullable), d_argmax[n] (nullable), *d_vsum += sum(argmax + 1) (nullable).
+ * A sample whose material is outside [0, 12) is skipped (argmax -1, adds
+ * nothing); on the hash grid energies outside [0, 1) use the nearest bin.
  * Enqueued on `stream` (NULL = the handle's stream); does not synchronize. */
 int xsb_hip_lookup(void *handle, int64_t n, const double *d_energy, const int32_t *d_mat,
                    int32_t xs_precision, double *d_macro, int32_t *d_argmax,

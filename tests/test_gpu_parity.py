@@ -267,14 +267,20 @@ def _hip_runtime():
     return hip
 
 
-def test_lookup_on_device_buffers(orc):
-    inp, sd, dev = data("small_hash")
+@pytest.mark.parametrize("name", ["small_hash", "small_unionized"])
+def test_lookup_on_device_buffers(orc, name):
+    """explicit samples in device memory; samples with a material outside
+    [0, 12) are skipped (argmax -1), energies outside [0, 1) stay in bounds"""
+    inp, sd, dev = data(name)
     oi, osd = as_oracle(inp, sd)
     hip = _hip_runtime()
     rng = np.random.default_rng(17)
     n = 50000
     E = rng.random(n)
     M = rng.integers(0, 12, n).astype(np.int32)
+    bad = np.arange(8) * 997
+    M[bad[:4]] = [-1, 12, 1 << 30, -(1 << 30)]
+    E[bad[4:]] = [-0.5, np.nan, 1.5, 1.0]
     ptrs = [C.c_void_p() for _ in range(5)]
     sizes = [n * 8, n * 4, n * 40, n * 4, 8]
     for p_, b in zip(ptrs, sizes):
@@ -295,10 +301,14 @@ def test_lookup_on_device_buffers(orc):
     finally:
         for p_ in ptrs:
             hip.hipFree(p_)
-    want_macro, want_am = orc.lookup_detail(oi, osd, E, M)
-    assert same_bits(macro, want_macro)
-    assert np.array_equal(am, want_am)
-    assert int(vs[0]) == int(want_am.sum()) + n
+    assert np.all(am[bad[:4]] == -1)
+    ok = np.ones(n, bool)
+    ok[bad] = False
+    want_macro, want_am = orc.lookup_detail(oi, osd, E[ok], M[ok])
+    assert same_bits(macro[ok], want_macro)
+    assert np.array_equal(am[ok], want_am)
+    assert np.all((am[bad[4:]] >= 0) & (am[bad[4:]] < 5))
+    assert int(vs[0]) == int(am[am >= 0].sum()) + int((am >= 0).sum())
 
 
 def _batch_truth(orc, oi, osd, sizes, seeds, thr):

@@ -218,17 +218,19 @@ __device__ __forceinline__ int32_t nuclide_lower(const Tables &T, const double *
 
 // Grid-level index for a lookup: unionized-grid bracket or hash bin
 // (openmp-threading/Simulation.c:346-352).  The hash bin keeps the
-// reference's division p / (1.0 / bins); it is clamped to the last bin so
-// that p == 1.0 (which the reference would read past its table with) stays
-// in bounds.
+// reference's division p / (1.0 / bins) and truncation; it is clamped to
+// [0, bins - 1] so that p == 1.0 (which the reference would read past its
+// table with), or a caller's negative or NaN energy, stays in bounds.
 template <int GRID>
 __device__ __forceinline__ int64_t lookup_index(const Tables &T, double p)
 {
 	if (GRID == kUnionized)
 		return grid_search((int32_t)T.n_union, p, T.ueg);
 	if (GRID == kHash) {
-		int32_t b = (int32_t)(p / T.hash_du);
-		return b < T.hash_bins ? b : T.hash_bins - 1;
+		const double q = p / T.hash_du;
+		if (!(q >= 0.0))
+			return 0;
+		return q < (double)(T.hash_bins - 1) ? (int32_t)q : T.hash_bins - 1;
 	}
 	return -1;
 }
@@ -379,7 +381,7 @@ __device__ __forceinline__ void macro_ueg_wave(const Tables &T, double p, int m,
 	});
 }
 
-constexpr int kIdxChunk = 16;
+constexpr int kIdxChunk = 8;
 
 // Hash grid: the bracket of one nuclide for a lookup in bin idx
 // (Simulation.c:273-302), with the bin's bounds u_low / u_high given.  When

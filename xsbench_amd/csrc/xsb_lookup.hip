@@ -116,7 +116,12 @@ __global__ __launch_bounds__(kBlock) void k_lookup_list(Tables T, int64_t n, con
 {
 	const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
 	uint32_t v = 0, fb = 0;
-	if (i < n) {
+	// a material outside [0, 12) is not looked up: it adds nothing and its
+	// argmax output is -1
+	if (i < n && (unsigned)M[i] >= (unsigned)kMaterials) {
+		if (DETAIL && det.argmax)
+			det.argmax[perm ? (int64_t)perm[i] : i] = -1;
+	} else if (i < n) {
 		const double p = E[i];
 		const int mat = M[i];
 		const int64_t idx = lookup_index<GRID>(T, p);
