@@ -139,8 +139,8 @@ typedef struct {
 	int32_t tuned;     /* 1 if the autotuner ran during this call */
 	int32_t graph;     /* 1 if the pipeline ran as a hipGraph */
 	int32_t mapping;   /* 1: one lane per lookup; 2: wave-uniform path where a wave's
-	                      lanes share the material (unionized grid); 3: as 2 without
-	                      the three-point shared step (unionized FP64) */
+	                      lanes share the material (and on the hash grid the bin);
+	                      3: as 2 without the three-point shared step */
 	int32_t reserved0;
 	double ms_tuning;  /* host wall time spent tuning during this call */
 } xsb_batch_plan;

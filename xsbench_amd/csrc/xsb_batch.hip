@@ -454,14 +454,14 @@ static int launch_or_size(Device *d, const BatchView &B, const SortedQueue &q, b
 }
 
 // dispatch on (grid type, precision, staging depth, three-point step); the
-// last two only matter for the unionized FP64 wave path, the others use one
-// instantiation
+// staging depth only matters on the unionized grid
 template <int GRID, int PREC>
 static int dispatch_bsorted(Device *d, const BatchView &B, const SortedQueue &q, int stage,
                             bool share3, bool occupancy_only, int *occ, hipStream_t s)
 {
-	if (!(PREC == XSB_XS_FP64 && GRID == kUnionized))
-		return launch_or_size<GRID, PREC, 16, true>(d, B, q, occupancy_only, occ, s);
+	if (GRID != kUnionized)
+		return share3 ? launch_or_size<GRID, PREC, 16, true>(d, B, q, occupancy_only, occ, s)
+		              : launch_or_size<GRID, PREC, 16, false>(d, B, q, occupancy_only, occ, s);
 	if (stage >= 16)
 		return share3 ? launch_or_size<GRID, PREC, 16, true>(d, B, q, occupancy_only, occ, s)
 		              : launch_or_size<GRID, PREC, 16, false>(d, B, q, occupancy_only, occ, s);
@@ -498,8 +498,8 @@ static int run_bsorted(Device *d, int prec, const BatchView &B, const SortedQueu
 
 // ---- host side of mode 0 ----
 // mapping: 1 one lane per lookup (the reference loop per lane), 2 the
-// wave-uniform path where a wave's lanes share the material, 3 the same
-// without the three-point shared step (unionized FP64)
+// wave-uniform path where a wave's lanes share the material (and, on the
+// hash grid, the bin), 3 the same without the three-point shared step
 struct BatchPlan {
 	int tile = 0, stage = 0, sort = 0, mapping = 0;
 	bool fits(const BatchPlan &fixed) const
@@ -690,7 +690,7 @@ static int time_pipeline(Device *d, int nb, int64_t n, int windows, int prec, co
 // Picks the plan fields left open in `fixed` by timing each candidate (twice,
 // keeping the faster) on a prefix of the batches of about 16 M lookups; a
 // prefix of a random set of batches keeps its size histogram.  The staging
-// depth and the mapping only matter on the unionized FP64 path.
+// depth only matters on the unionized grid.
 static int autotune(Device *d, const xsb_batched_args *a, const std::vector<uint32_t> &off,
                     const std::vector<uint32_t> &win, char *base, const TableLayout &L,
                     const BatchPlan &fixed, BatchPlan *plan)
@@ -705,13 +705,10 @@ static int autotune(Device *d, const xsb_batched_args *a, const std::vector<uint
 	const int64_t nt = off[nbt];
 	const int wt = (int)win[nbt - 1] + 1;
 	auto pick = [](int v, std::vector<int> all) { return v ? std::vector<int>{v} : all; };
-	const bool staged = d->T.grid_type == kUnionized && a->xs_precision == XSB_XS_FP64;
+	const bool staged = d->T.grid_type == kUnionized;
 	const std::vector<int> tiles = pick(fixed.tile, {2048, 8192});
 	const std::vector<int> sorts = pick(fixed.sort, {2, 1});
-	const std::vector<int> maps =
-	    staged ? pick(fixed.mapping, {2, 3, 1})
-	           : pick(fixed.mapping, d->T.grid_type == kUnionized ? std::vector<int>{2, 1}
-	                                                              : std::vector<int>{1});
+	const std::vector<int> maps = pick(fixed.mapping, {2, 3, 1});
 	float best = 1e30f;
 	BatchPlan best_plan;
 	for (int so : sorts)

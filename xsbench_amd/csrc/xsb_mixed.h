@@ -189,10 +189,11 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 	return mixed_finish<GRID, PREC, HISTORY>(T, p, mat, idx, sum, A, macro, fell_back, bound);
 }
 
-// One nuclide of the wave-uniform unionized path on the mixed tables,
-// bracket k per lane: records through the scalar cache when the lanes'
-// brackets coincide or span two neighbouring intervals, else per lane.
-template <int PREC>
+// One nuclide of the wave-uniform path on the mixed tables, bracket k per
+// lane: records through the scalar cache when the lanes' brackets coincide,
+// and with SHARE3 also when they span two neighbouring intervals, else per
+// lane (as ueg_wave_nuclide).
+template <int PREC, bool SHARE3 = true>
 __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc, int k, double p,
                                                        float c, float cb, float sum[5],
                                                        float comp[5], float &A)
@@ -210,7 +211,8 @@ __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc,
 		load_xs<PREC>(P, x0);
 		load_xs<PREC>(P + REC, x1);
 		mixed_term(P[0], x0, P[REC], x1, p, c, cb, sum, comp, A);
-	} else if (__ballot(d != 0 && d != 1) == 0ULL || __ballot(d != 0 && d != -1) == 0ULL) {
+	} else if (SHARE3 &&
+	           (__ballot(d != 0 && d != 1) == 0ULL || __ballot(d != 0 && d != -1) == 0ULL)) {
 		// three records Q, Q+1, Q+2 shared; each lane takes its pair
 		const double *Q = (__ballot(d == -1) != 0ULL) ? P - REC : P;
 		const int sel = (__ballot(d == -1) != 0ULL) ? d + 1 : d;
@@ -226,6 +228,7 @@ __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc,
 			xh[w] = sel ? x2[w] : x1[w];
 		}
 		mixed_term(sel ? e1 : e0, xl, sel ? e2 : e1, xh, p, c, cb, sum, comp, A);
+		__builtin_amdgcn_sched_barrier(0);
 	} else {
 		const double *L = row + (int64_t)k * REC;
 		load_xs<PREC>(L, x0);
@@ -236,7 +239,7 @@ __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc,
 
 // calculate_macro_xs, unionized grid, mixed tables, for a wave whose lanes
 // all hold material m; same terms and decision as macro_xs_mixed.
-template <int PREC, int CHUNK, bool HISTORY>
+template <int PREC, int CHUNK, bool HISTORY, bool SHARE3 = true>
 __device__ __forceinline__ int macro_ueg_wave_mixed(const Tables &T, double p, int m, int64_t idx,
                                                     double macro[5], uint32_t *fell_back,
                                                     double *bound)
@@ -247,14 +250,14 @@ __device__ __forceinline__ int macro_ueg_wave_mixed(const Tables &T, double p, i
 	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
 	float A = 0.f;
 	ueg_wave_rows<CHUNK>(T, m, idx, [&](int j, int nuc, int k) {
-		ueg_wave_nuclide_mixed<PREC>(T, nuc, k, p, ceff[j], cbnd[j], sum, comp, A);
+		ueg_wave_nuclide_mixed<PREC, SHARE3>(T, nuc, k, p, ceff[j], cbnd[j], sum, comp, A);
 	});
 	return mixed_finish<kUnionized, PREC, HISTORY>(T, p, m, idx, sum, A, macro, fell_back, bound);
 }
 
 // calculate_macro_xs, hash grid, mixed tables, for a wave whose lanes share
 // the material and the hash bin; same terms and decision as macro_xs_mixed.
-template <int PREC, bool HISTORY>
+template <int PREC, bool HISTORY, bool SHARE3 = true>
 __device__ __forceinline__ int macro_hash_wave_mixed(const Tables &T, double p, int m, int64_t idx,
                                                      double macro[5], uint32_t *fell_back,
                                                      double *bound)
@@ -266,7 +269,7 @@ __device__ __forceinline__ int macro_hash_wave_mixed(const Tables &T, double p, 
 	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
 	float A = 0.f;
 	hash_wave_rows<REC>(T, mixed_row<PREC>(T, 0), p, m, idx, [&](int j, int nuc, int k) {
-		ueg_wave_nuclide_mixed<PREC>(T, nuc, k, p, ceff[j], cbnd[j], sum, comp, A);
+		ueg_wave_nuclide_mixed<PREC, SHARE3>(T, nuc, k, p, ceff[j], cbnd[j], sum, comp, A);
 	});
 	return mixed_finish<kHash, PREC, HISTORY>(T, p, m, idx, sum, A, macro, fell_back, bound);
 }

@@ -59,8 +59,7 @@ __device__ __forceinline__ void write_tile_hist(const uint32_t *h, uint32_t *his
 // One lane's lookup (p, mat).  Unionized waves whose lanes share the
 // material take the wave-uniform path (macro_ueg_wave / _mixed, index words
 // staged CHUNK deep) when `wave` is set; everything else runs the per-lane
-// loop.  SHARE3 selects the FP64 wave path's three-point step
-// (ueg_wave_nuclide).  Returns the argmax channel; idx gets the grid-level index,
+// loop.  SHARE3 selects the wave paths' three-point step (ueg_wave_nuclide).  Returns the argmax channel; idx gets the grid-level index,
 // macro the sums, fb the fallback flag (mixed precision).
 // HISTORY: the caller also tests macro_c > 1.0, so mixed precision falls
 // back to FP64 near 1.0 too and then returns the FP64 sums.
@@ -86,12 +85,13 @@ __device__ __forceinline__ int sorted_lookup(const Tables &T, double p, int mat,
 				macro_hash_wave<SHARE3>(T, p, m0, i0, macro);
 				return argmax5(macro);
 			}
-			return macro_hash_wave_mixed<PREC, HISTORY>(T, p, m0, i0, macro, &fb, bnd);
+			return macro_hash_wave_mixed<PREC, HISTORY, SHARE3>(T, p, m0, i0, macro, &fb, bnd);
 		}
 	} else if (GRID == kUnionized && PREC != XSB_XS_FP64) {
 		const int m0 = __builtin_amdgcn_readfirstlane(mat);
 		if (wave && __ballot(mat != m0) == 0ULL)
-			return macro_ueg_wave_mixed<PREC, CHUNK, HISTORY>(T, p, m0, idx, macro, &fb, bnd);
+			return macro_ueg_wave_mixed<PREC, CHUNK, HISTORY, SHARE3>(T, p, m0, idx, macro, &fb,
+			                                                          bnd);
 	}
 	if (PREC == XSB_XS_FP64) {
 		macro_xs_fp64<GRID>(T, p, mat, idx, macro);
