@@ -112,7 +112,7 @@ static void release(Device *d)
 	batch_state_free(d);
 	if (d->history_graph.exec)
 		(void)hipGraphExecDestroy(d->history_graph.exec);
-	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_midx, d->d_grid,  d->d_egrid,    d->d_xs32, d->d_xs16,
+	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_midx, d->d_grid,  d->d_xs32, d->d_xs16,
 	                d->d_xs16_scale, d->d_num_nucs, d->d_mats, d->d_concs, d->d_concs32, d->d_acc,
 	                d->d_fftab};
 	for (void *p : ptrs)

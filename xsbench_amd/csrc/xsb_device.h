@@ -26,7 +26,6 @@ struct Tables {
 	const double *__restrict__ ueg;         // unionized energies [n_union]
 	const int32_t *__restrict__ index_grid; // [n_union or hash_bins][n_iso]
 	const double *__restrict__ grid;        // FP64 AoS, 6 doubles per point
-	const double *__restrict__ egrid;       // FP64 energies only [n_iso*n_gp] (mixed tables)
 	const float *__restrict__ xs32;         // FP32 XS, 8 floats per point (5 used)
 	const uint16_t *__restrict__ xs16;      // BF16 XS, 8 halves per point (5 used)
 	const float *__restrict__ xs16_scale;   // per nuclide x channel scale (5 per nuclide, 8 stride)
@@ -131,21 +130,6 @@ __device__ __forceinline__ int pick_mat(uint64_t &s, const double *thr)
 __device__ __forceinline__ int32_t grid_search(int32_t n, double q, const double *__restrict__ A)
 {
 	int32_t lo = 0, hi = n - 1;
-	while (hi - lo > 1) {
-		int32_t mid = lo + ((hi - lo) >> 1);
-		if (A[mid] > q)
-			hi = mid;
-		else
-			lo = mid;
-	}
-	return lo;
-}
-
-// The reference's bisection restricted to [lo, hi]: returns the same index
-// as grid_search whenever that index is known to lie in [lo, hi - 1].
-__device__ __forceinline__ int32_t grid_search_range(int32_t lo, int32_t hi, double q,
-                                                     const double *__restrict__ A)
-{
 	while (hi - lo > 1) {
 		int32_t mid = lo + ((hi - lo) >> 1);
 		if (A[mid] > q)

@@ -49,7 +49,6 @@ struct Device {
 	int32_t *d_index = nullptr;
 	int32_t *d_midx = nullptr;  // per-material unionized index tables
 	double *d_grid = nullptr;
-	double *d_egrid = nullptr;
 	float *d_xs32 = nullptr;
 	uint16_t *d_xs16 = nullptr;
 	float *d_xs16_scale = nullptr;
