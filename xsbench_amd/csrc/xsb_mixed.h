@@ -105,11 +105,13 @@ __device__ __forceinline__ void mixed_term(double e_lo, const float xl[5], doubl
 
 // The decision on the mixed sums (see the bound above): returns the argmax
 // the FP64 reference would return; macro[] receives the mixed-precision sums,
-// or the FP64 sums when the lookup fell back (*fell_back = 1).
-template <int GRID, int PREC, bool HISTORY>
-__device__ __forceinline__ int mixed_finish(const Tables &T, double p, int mat, int64_t idx,
-                                            const float sum[5], float A, double macro[5],
-                                            uint32_t *fell_back, double *bound)
+// or the FP64 sums when the lookup fell back (*fell_back = 1).  fp64(m)
+// writes this lane's FP64 sums.  WAVE: fp64 is a wave-cooperative path that
+// every lane of the wave must enter, so it runs for the whole wave as soon as
+// one lane needs it (cheaper than one lane's FP64 loop with its wave idle).
+template <int PREC, bool HISTORY, bool WAVE, class F64>
+__device__ __forceinline__ int mixed_finish(const float sum[5], float A, double macro[5],
+                                            uint32_t *fell_back, double *bound, F64 &&fp64)
 {
 	constexpr float kU = (PREC == XSB_XS_FP32) ? 0x1p-24f : 0x1p-8f;
 	// argmax of the mixed sums (max = -1.0, strict '>', as the reference)
@@ -117,7 +119,7 @@ __device__ __forceinline__ int mixed_finish(const Tables &T, double p, int mat, 
 #pragma unroll
 	for (int k = 0; k < 5; k++)
 		s[k] = (double)sum[k];
-	const int am = argmax5(s);
+	int am = argmax5(s);
 	const double B = 1.01 * (double)A * ((double)kU + 12.0 * 0x1p-24);
 	if (bound)
 		*bound = B;
@@ -140,17 +142,18 @@ __device__ __forceinline__ int mixed_finish(const Tables &T, double p, int mat, 
 #pragma unroll
 	for (int k = 0; k < 5; k++)
 		macro[k] = s[k];
-	if (!ambiguous) {
-		*fell_back = 0;
+	*fell_back = ambiguous ? 1u : 0u;
+	if (WAVE ? __ballot(ambiguous) == 0ULL : !ambiguous)
 		return am;
-	}
-	*fell_back = 1;
 	double m64[5];
-	macro_xs_fp64<GRID>(T, p, mat, idx, m64);
+	fp64(m64);
+	if (ambiguous) {
 #pragma unroll
-	for (int k = 0; k < 5; k++)
-		macro[k] = m64[k];
-	return argmax5(m64);
+		for (int k = 0; k < 5; k++)
+			macro[k] = m64[k];
+		am = argmax5(m64);
+	}
+	return am;
 }
 
 template <int PREC>
@@ -186,7 +189,9 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 		load_xs<PREC>(hi, xh);
 		mixed_term(lo[0], xl, hi[0], xh, p, ceff[j], cbnd[j], sum, comp, A);
 	}
-	return mixed_finish<GRID, PREC, HISTORY>(T, p, mat, idx, sum, A, macro, fell_back, bound);
+	return mixed_finish<PREC, HISTORY, false>(sum, A, macro, fell_back, bound, [&](double m64[5]) {
+		macro_xs_fp64<GRID>(T, p, mat, idx, m64);
+	});
 }
 
 // One nuclide of the wave-uniform path on the mixed tables, bracket k per
@@ -252,7 +257,9 @@ __device__ __forceinline__ int macro_ueg_wave_mixed(const Tables &T, double p, i
 	ueg_wave_rows<CHUNK>(T, m, idx, [&](int j, int nuc, int k) {
 		ueg_wave_nuclide_mixed<PREC, SHARE3>(T, nuc, k, p, ceff[j], cbnd[j], sum, comp, A);
 	});
-	return mixed_finish<kUnionized, PREC, HISTORY>(T, p, m, idx, sum, A, macro, fell_back, bound);
+	return mixed_finish<PREC, HISTORY, true>(sum, A, macro, fell_back, bound, [&](double m64[5]) {
+		macro_ueg_wave<CHUNK, SHARE3>(T, p, m, idx, m64);
+	});
 }
 
 // calculate_macro_xs, hash grid, mixed tables, for a wave whose lanes share
@@ -271,7 +278,9 @@ __device__ __forceinline__ int macro_hash_wave_mixed(const Tables &T, double p, 
 	hash_wave_rows<REC>(T, mixed_row<PREC>(T, 0), p, m, idx, [&](int j, int nuc, int k) {
 		ueg_wave_nuclide_mixed<PREC, SHARE3>(T, nuc, k, p, ceff[j], cbnd[j], sum, comp, A);
 	});
-	return mixed_finish<kHash, PREC, HISTORY>(T, p, m, idx, sum, A, macro, fell_back, bound);
+	return mixed_finish<PREC, HISTORY, true>(sum, A, macro, fell_back, bound, [&](double m64[5]) {
+		macro_hash_wave<SHARE3>(T, p, m, idx, m64);
+	});
 }
 
 }  // namespace xsb
