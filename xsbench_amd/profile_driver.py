@@ -22,6 +22,8 @@ WORKLOADS = {
     "large_hash": ("large", "hash", "event", None),
     "small_unionized": ("small", "unionized", "event", None),
     "xl45k_hash_history": ("XL", "hash", "history", 45000),
+    # 512 batches, sizes log-uniform in [1e3, 2e6] (xsbench_make_batches seed 2024)
+    "large_unionized_batched": ("large", "unionized", "batched", None),
 }
 
 COPY_BYTES = 1 << 30
@@ -36,6 +38,8 @@ def main():
     ap.add_argument("--lookups", type=int, default=17_000_000)
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--meta", default=None, help="write workload facts as JSON here")
+    ap.add_argument("--plan", default="2048,16,2,3",
+                    help="batched: tile,stage,sort,mapping (fixed, so no tuning runs are traced)")
     args = ap.parse_args()
     size, grid, method, gp = WORKLOADS[args.workload]
     inp = X.make_inputs(size, grid, "event", gridpoints=gp)
@@ -43,9 +47,15 @@ def main():
     prec = X.PRECISIONS[args.precision]
     dev = X.Device(inp, sd, build_fp32=prec == X.XS_FP32, build_bf16=prec == X.XS_BF16)
     ms = []
+    if method == "batched":
+        sizes, seeds, thr = X.make_batches(2024, 512, 1000, 2_000_000)
+        tile, stage, sort, mapping = (int(v) for v in args.plan.split(","))
     for _ in range(args.reps):
         if method == "event":
             r = dev.event(args.lookups, 0, kernel_id=args.kernel, precision=prec)
+        elif method == "batched":
+            r, _, _ = dev.batched(sizes, seeds, thr, precision=prec, mode=0, tile=tile,
+                                  stage=stage, sort=sort, mapping=mapping)
         else:
             r = dev.history(args.particles, 34, kernel_id=args.kernel, precision=prec)
         ms.append(r["ms_lookup"])
