@@ -372,6 +372,27 @@ def test_batched_graph_replays_match_eager(orc):
     assert [int(v) for v in vs2] == _batch_truth(orc, oi, osd, sizes, seeds2, thr)
 
 
+def test_batched_autotuner_caches_its_plan(orc):
+    """the first call with open plan fields tunes; a second call of the same
+    shape reuses the cached plan and the captured graph; fixing one field
+    tunes the rest again"""
+    inp, sd, dev = data("small_hash")
+    oi, osd = as_oracle(inp, sd)
+    sizes, seeds, thr = orc.batch_plan(31, 20, 500, 20000)
+    want = _batch_truth(orc, oi, osd, sizes, seeds, thr)
+    p1, p2, p3 = {}, {}, {}
+    _, v1, _ = dev.batched(sizes, seeds, thr, plan=p1)
+    _, v2, _ = dev.batched(sizes, seeds, thr, plan=p2)
+    assert p1["tuned"] == 1 and p2["tuned"] == 0 and p2["graph"] == 1
+    assert {k: p1[k] for k in ("tile", "stage", "sort", "mapping")} == \
+           {k: p2[k] for k in ("tile", "stage", "sort", "mapping")}
+    other = 1 if p1["mapping"] != 1 else 2
+    _, v3, _ = dev.batched(sizes, seeds, thr, mapping=other, plan=p3)
+    assert p3["mapping"] == other and p3["tuned"] == 1
+    for v in (v1, v2, v3):
+        assert [int(x) for x in v] == want
+
+
 def test_batched_many_batches_global_sums(orc):
     """more batches than the kernel's LDS table: per-batch sums go to device atomics"""
     inp, sd, dev = data("small_hash")

@@ -11,8 +11,8 @@ namespace xsb {
 // ---- radix sort of (key, value) pairs ----
 constexpr int kSortBlock = 256;
 constexpr int kSortWaves = kSortBlock / 64;
-constexpr int kSortItems = 16;
-constexpr int kTile = kSortBlock * kSortItems;  // 4096 elements per tile
+constexpr int kSortItems = 32;
+constexpr int kTile = kSortBlock * kSortItems;  // 8192 elements per tile
 constexpr int kDigitBits = 10;
 constexpr int kBins = 1 << kDigitBits;
 constexpr uint32_t kDigitMask = kBins - 1;
