@@ -314,8 +314,10 @@ __global__ void k_digit_starts(const uint32_t *__restrict__ hist, int ntiles, ui
 // re-derived from the id's LCG state.  Lanes past n repeat lookup n-1 (in the
 // same wave) and contribute nothing, so every lane stays active for the
 // wave-wide search and the wave-uniform material path.
+// Eight waves per SIMD (at most 64 VGPRs, a few spilled) hide the per-nuclide
+// latency chain better than the six the register demand would otherwise give.
 template <int GRID, int PREC, bool DETAIL>
-__global__ __launch_bounds__(256) void k_lookup_ids(Tables T, int64_t n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_lookup_ids(Tables T, int64_t n,
                                                     const uint32_t *__restrict__ ids,
                                                     int64_t first, uint64_t seed0, DetailOut det,
                                                     unsigned long long *vsum,
