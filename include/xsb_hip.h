@@ -46,6 +46,9 @@ extern "C" {
                                 on the device (GridInit.c:3-151) instead of copying them:
                                 only n_isotopes, n_gridpoints, grid_type, hash_bins and the
                                 material tables of xsb_tables are read */
+#define XSB_BUILD_IDX32 0x8u /* keep the per-material unionized index tables 32-bit; by default
+                                they are 16-bit whenever n_gridpoints <= 65536 (same results,
+                                half the index traffic) */
 
 /* ---- lookup schedules: the reference's -k kernel ids
  *      (openmp-threading/Main.c:80-93, cuda/Main.cu:62-83) ---- */

@@ -206,6 +206,63 @@ def test_device_grid_construction_tiny_grids():
 
 
 # ---------------------------------------------------------------------------
+# per-material unionized index tables: uint16 brackets when n_gridpoints <=
+# 65536 (the default), int32 otherwise or on request (BUILD_IDX32)
+# ---------------------------------------------------------------------------
+def _midx_bytes(dev, n_union, width):
+    """bytes of the per-material index tables: rows padded to 16 bytes"""
+    nn = np.ctypeslib.as_array(C.cast(dev.tables.num_nucs, C.POINTER(C.c_int)), (12,))
+    per = 16 // width
+    return sum(n_union * ((int(v) + per - 1) // per * per) * width for v in nn)
+
+
+@pytest.mark.parametrize("name", ["small_unionized", "large_unionized"])
+def test_index_tables_32bit_match_oracle(orc, name):
+    inp, sd, dev16 = data(name)
+    oi, osd = as_oracle(inp, sd)
+    n_union = inp.n_isotopes * inp.n_gridpoints
+    assert dev16.info()["bytes_midx"] == _midx_bytes(dev16, n_union, 2)
+    want = X.PUBLISHED[("event", DATA_CFGS[name][0])]
+    print(f"[gpu-test] 32-bit index tables {name}", flush=True)
+    with X.Device(inp, sd, build_fp32=True, idx32=True) as dev:
+        assert dev.info()["bytes_midx"] == _midx_bytes(dev, n_union, 4)
+        first, n = 5_000_003, 1 << 15
+        E, m, idx, macro, am = orc.event_detail(oi, osd, np.arange(first, first + n))
+        for k in (X.K_BASELINE, X.K_MAT_E_SORT):
+            _, out = dev.event(n, first, kernel_id=k, detail=True)
+            assert same_bits(out["macro"], macro) and np.array_equal(out["argmax"], am)
+        for k in (X.K_BASELINE, X.K_MAT_E_SORT):
+            assert dev.event(17_000_000, kernel_id=k)["hash"] == want
+        assert dev.event(17_000_000, kernel_id=X.K_MAT_E_SORT, precision=X.XS_FP32)["hash"] == want
+        _, h16 = dev16.history(20000, 34, kernel_id=6, detail=True)
+        _, h32 = dev.history(20000, 34, kernel_id=6, detail=True)
+        assert np.array_equal(h16["sums"], h32["sums"])
+        sizes, seeds, thr = orc.batch_plan(2024, 24, 1000, 60000)
+        _, v16, _ = dev16.batched(sizes, seeds, thr, mode=0)
+        _, v32, _ = dev.batched(sizes, seeds, thr, mode=0)
+        assert np.array_equal(v16, v32)
+
+
+def test_index_tables_width_boundary():
+    """n_gridpoints 65536 (largest uint16 case, brackets up to 65534) and
+    65537 (int32 chosen by itself): equal to forced int32 tables"""
+    for gp, width in ((65536, 2), (65537, 4)):
+        inp = X.make_inputs("small", "unionized", "event", gridpoints=gp)
+        n_union = inp.n_isotopes * gp
+        print(f"[gpu-test] index width boundary gridpoints={gp}", flush=True)
+        with X.Device(inp, None) as a, X.Device(inp, None, idx32=True) as b:
+            assert a.info()["bytes_midx"] == _midx_bytes(a, n_union, width)
+            assert b.info()["bytes_midx"] == _midx_bytes(b, n_union, 4)
+            for k in (X.K_BASELINE, X.K_MAT_E_SORT):
+                _, oa = a.event(1 << 16, 12345, kernel_id=k, detail=True)
+                _, ob = b.event(1 << 16, 12345, kernel_id=k, detail=True)
+                assert np.array_equal(oa["index"], ob["index"]) and oa["index"].max() > 0.99 * n_union
+                assert same_bits(oa["macro"], ob["macro"])
+            assert (a.event(1_000_000, kernel_id=X.K_MAT_E_SORT)["hash"]
+                    == b.event(1_000_000, kernel_id=X.K_MAT_E_SORT)["hash"])
+
+
+# ---------------------------------------------------------------------------
 # mixed precision: identical decisions, bounded error
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("prec", [X.XS_FP32, X.XS_BF16])

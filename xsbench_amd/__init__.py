@@ -32,7 +32,7 @@ GRID_TYPES = {"unionized": UNIONIZED, "nuclide": NUCLIDE, "hash": HASH}
 
 XS_FP64, XS_FP32, XS_BF16 = 0, 1, 2
 PRECISIONS = {"fp64": XS_FP64, "fp32": XS_FP32, "bf16": XS_BF16}
-BUILD_FP32, BUILD_BF16, BUILD_GRIDS = 0x1, 0x2, 0x4
+BUILD_FP32, BUILD_BF16, BUILD_GRIDS, BUILD_IDX32 = 0x1, 0x2, 0x4, 0x8
 
 # lookup schedules = the reference's -k kernel ids (cuda/Main.cu:62-83)
 K_BASELINE, K_SPLIT, K_MAT_LAUNCH, K_FUEL_LAUNCH, K_MAT_SORT, K_FUEL_SORT, K_MAT_E_SORT = range(7)
@@ -369,11 +369,12 @@ class Device:
     With sd=None the grids are generated on the device (xsb_hip_upload with
     BUILD_GRIDS): only the material tables come from the host."""
 
-    def __init__(self, inp, sd, device=0, build_fp32=False, build_bf16=False):
+    def __init__(self, inp, sd, device=0, build_fp32=False, build_bf16=False, idx32=False):
         self.lib = hip_lib()
         self.inp = inp
         self.tables = tables_from_sd(inp, sd) if sd is not None else material_tables(inp)
         flags = (BUILD_FP32 if build_fp32 else 0) | (BUILD_BF16 if build_bf16 else 0)
+        flags |= BUILD_IDX32 if idx32 else 0
         if sd is None:
             flags |= BUILD_GRIDS
         h = C.c_void_p()

@@ -416,32 +416,39 @@ static int download(Device *d, double *grid, double *ueg, int32_t *index)
 }
 
 // Material m's unionized index table: row e holds index_grid[e][mats[m][j]]
-// for j < nn, then zeros up to the pitch mp.
+// for j < nn, then zeros up to the pitch mp; IDX is the element type.
+template <class IDX>
 __global__ void k_material_index(const int32_t *__restrict__ index, int64_t rows, int32_t pitch,
                                  const int32_t *__restrict__ ml, int32_t nn, int32_t mp,
-                                 int32_t *__restrict__ dst)
+                                 IDX *__restrict__ dst)
 {
 	const int64_t total = rows * mp;
 	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
 	     t += (int64_t)gridDim.x * blockDim.x) {
 		const int64_t e = t / mp;
 		const int j = (int)(t - e * mp);
-		dst[t] = (j < nn) ? index[e * pitch + ml[j]] : 0;
+		dst[t] = (IDX)((j < nn) ? index[e * pitch + ml[j]] : 0);
 	}
 }
 
 // Per-material copies of the unionized index grid (Tables::midx): each
 // lookup then reads its material's brackets from one contiguous row.
-static int build_material_index(Device *d, const xsb_tables *t, int64_t rows, int64_t pitch)
+// Brackets are at most n_gp - 1, so they are stored as uint16 when
+// n_gp <= 65536 unless `wide` asks for int32.
+static int build_material_index(Device *d, const xsb_tables *t, int64_t rows, int64_t pitch,
+                                bool wide)
 {
 	Tables &T = d->T;
+	const bool narrow = !wide && T.n_gp <= 65536;
+	T.midx_shift = narrow ? 1 : 2;
+	const int align = narrow ? 8 : 4;  // entries per 16 bytes
 	int64_t total = 0;
 	for (int m = 0; m < kMaterials; m++) {
 		T.midx_off[m] = total;
-		T.midx_pitch[m] = (t->num_nucs[m] + 3) & ~3;
+		T.midx_pitch[m] = (t->num_nucs[m] + align - 1) & ~(align - 1);
 		total += rows * T.midx_pitch[m];
 	}
-	const size_t bytes = (size_t)total * sizeof(int32_t);
+	const size_t bytes = (size_t)total << T.midx_shift;
 	if (hipMalloc((void **)&d->d_midx, bytes) != hipSuccess) {
 		(void)hipGetLastError();
 		d->d_midx = nullptr;
@@ -451,9 +458,15 @@ static int build_material_index(Device *d, const xsb_tables *t, int64_t rows, in
 	for (int m = 0; m < kMaterials; m++) {
 		if (!T.midx_pitch[m])
 			continue;
-		hipLaunchKernelGGL(k_material_index, dim3(4096), dim3(256), 0, d->stream, d->d_index, rows,
-		                   (int32_t)pitch, d->d_mats + (int64_t)m * t->max_num_nucs,
-		                   t->num_nucs[m], T.midx_pitch[m], d->d_midx + T.midx_off[m]);
+		const int32_t *ml = d->d_mats + (int64_t)m * t->max_num_nucs;
+		if (narrow)
+			hipLaunchKernelGGL(k_material_index<uint16_t>, dim3(4096), dim3(256), 0, d->stream,
+			                   d->d_index, rows, (int32_t)pitch, ml, t->num_nucs[m],
+			                   T.midx_pitch[m], (uint16_t *)d->d_midx + T.midx_off[m]);
+		else
+			hipLaunchKernelGGL(k_material_index<int32_t>, dim3(4096), dim3(256), 0, d->stream,
+			                   d->d_index, rows, (int32_t)pitch, ml, t->num_nucs[m],
+			                   T.midx_pitch[m], (int32_t *)d->d_midx + T.midx_off[m]);
 		XSB_HIP(hipGetLastError());
 	}
 	XSB_HIP(hipStreamSynchronize(d->stream));
@@ -573,12 +586,13 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	T.n_iso = (int32_t)n_iso;
 	T.idx_pitch = (int32_t)pitch;
 	T.variant = kVariantDefault;
-	if (t->grid_type == XSB_UNIONIZED && (rc = build_material_index(d, t, n_union, pitch)))
-		return rc;
 	T.n_gp = (int32_t)n_gp;
 	T.hash_bins = t->hash_bins;
 	T.max_num_nucs = t->max_num_nucs;
 	T.grid_type = t->grid_type;
+	if (t->grid_type == XSB_UNIONIZED &&
+	    (rc = build_material_index(d, t, n_union, pitch, (flags & XSB_BUILD_IDX32) != 0)))
+		return rc;
 	T.hash_du = (t->hash_bins > 0) ? 1.0 / t->hash_bins : 0.0;
 	pick_thresholds(T.thr);
 	for (int i = 1; i < kMaterials; i++)

@@ -45,11 +45,14 @@ struct Tables {
 	double thr[kMaterials];  // pick_mat thresholds, reverse-order partial sums
 	double hash_du;          // 1.0 / hash_bins, as computed by the reference
 	// unionized grid, per material m: its own copy of the index grid holding
-	// only its nuclides' columns, in material-list order, rows padded to a
-	// multiple of 4 ints -- row e at midx + midx_off[m] + e * midx_pitch[m]
-	const int32_t *__restrict__ midx;
+	// only its nuclides' columns, in material-list order, rows padded to 16
+	// bytes -- row e starts at element midx_off[m] + e * midx_pitch[m].
+	// Elements are 2 << (midx_shift - 1) bytes: uint16 (midx_shift 1) when
+	// every bracket fits (n_gp <= 65536), else int32 (midx_shift 2).
+	const uint8_t *__restrict__ midx;
 	int64_t midx_off[kMaterials];
 	int32_t midx_pitch[kMaterials];
+	int32_t midx_shift;
 };
 
 // ---------------------------------------------------------------------------
@@ -163,22 +166,29 @@ __device__ __forceinline__ int32_t grid_search_nuclide(double q, const double *_
 // ---------------------------------------------------------------------------
 // Unionized grid: row idx of material m's index table (entry j = the
 // bracket of the material's j-th nuclide).
-__device__ __forceinline__ const int32_t *ueg_row(const Tables &T, int m, int64_t idx)
+__device__ __forceinline__ const uint8_t *ueg_row(const Tables &T, int m, int64_t idx)
 {
-	return T.midx + T.midx_off[m] + idx * T.midx_pitch[m];
+	return T.midx + ((T.midx_off[m] + idx * T.midx_pitch[m]) << T.midx_shift);
+}
+
+// Entry j of an index row of either element width.
+__device__ __forceinline__ int32_t ueg_bracket(const Tables &T, const uint8_t *row, int j)
+{
+	return T.midx_shift == 1 ? (int32_t)reinterpret_cast<const uint16_t *>(row)[j]
+	                         : reinterpret_cast<const int32_t *>(row)[j];
 }
 
 // mrow: the material's unionized index row (ueg_row), j: the nuclide's
 // position in the material list; both used on the unionized grid only.
 template <int GRID, int STRIDE>
 __device__ __forceinline__ int32_t nuclide_lower(const Tables &T, const double *__restrict__ E_row,
-                                                 int nuc, const int32_t *__restrict__ mrow, int j,
+                                                 int nuc, const uint8_t *__restrict__ mrow, int j,
                                                  int64_t idx, double p)
 {
 	const int32_t n_gp = T.n_gp;
 	int32_t lower;
 	if (GRID == kUnionized) {
-		lower = mrow[j];
+		lower = ueg_bracket(T, mrow, j);
 	} else if (GRID == kNuclide) {
 		lower = grid_search_nuclide<STRIDE>(p, E_row, 0, n_gp - 1);
 	} else {
@@ -247,7 +257,7 @@ __device__ __forceinline__ void macro_xs_fp64(const Tables &T, double p, int mat
 	const int nn = T.num_nucs[mat];
 	const int32_t *__restrict__ ml = T.mats + mat * T.max_num_nucs;
 	const double *__restrict__ cl = T.concs + mat * T.max_num_nucs;
-	const int32_t *__restrict__ mrow = (GRID == kUnionized) ? ueg_row(T, mat, idx) : nullptr;
+	const uint8_t *__restrict__ mrow = (GRID == kUnionized) ? ueg_row(T, mat, idx) : nullptr;
 	for (int j = 0; j < nn; j++) {
 		const int nuc = ml[j];
 		const double conc = cl[j];
@@ -311,21 +321,44 @@ __device__ __forceinline__ void ueg_wave_nuclide(const Tables &T, int nuc, int k
 // Walks material m's nuclides in list order for the wave-uniform unionized
 // paths, calling f(j, nuclide, bracket k of this lane) for each.  The lane's
 // brackets are contiguous in its row of the material's index table, so they
-// are fetched as 16-byte vectors, CHUNK (the staging depth, a multiple of 4)
-// at a time, all in flight together.
+// are fetched as 16-byte vectors (8 uint16 or 4 int32 brackets each), CHUNK
+// (the staging depth, a multiple of 8) at a time, all in flight together.
+// The element width is a wave-uniform branch per chunk.
+// Eight uint16 brackets of one 16-byte vector as two int4 of brackets.
+__device__ __forceinline__ void unpack_u16x8(const int4 w, int4 &lo, int4 &hi)
+{
+	lo = make_int4((int)((uint32_t)w.x & 0xFFFFu), (int)((uint32_t)w.x >> 16),
+	               (int)((uint32_t)w.y & 0xFFFFu), (int)((uint32_t)w.y >> 16));
+	hi = make_int4((int)((uint32_t)w.z & 0xFFFFu), (int)((uint32_t)w.z >> 16),
+	               (int)((uint32_t)w.w & 0xFFFFu), (int)((uint32_t)w.w >> 16));
+}
+
 template <int CHUNK, class F>
 __device__ __forceinline__ void ueg_wave_rows(const Tables &T, int m, int64_t idx, F &&f)
 {
-	static_assert(CHUNK % 4 == 0, "staging depth is whole 16-byte vectors");
+	static_assert(CHUNK % 8 == 0, "staging depth is whole 16-byte vectors of either width");
 	const int nn = T.num_nucs[m];
 	const int32_t *__restrict__ ml = T.mats + m * T.max_num_nucs;
-	const int4 *__restrict__ row = reinterpret_cast<const int4 *>(ueg_row(T, m, idx));
+	const uint8_t *__restrict__ row = ueg_row(T, m, idx);
+	const bool narrow = T.midx_shift == 1;
 	int j0 = 0;
 	for (; j0 + CHUNK <= nn; j0 += CHUNK) {
 		int4 kk[CHUNK / 4];
+		if (narrow) {
+			const int4 *__restrict__ v = reinterpret_cast<const int4 *>(row) + j0 / 8;
+			int4 w[CHUNK / 8];
 #pragma unroll
-		for (int u = 0; u < CHUNK / 4; u++)
-			kk[u] = row[j0 / 4 + u];
+			for (int u = 0; u < CHUNK / 8; u++)
+				w[u] = v[u];
+#pragma unroll
+			for (int u = 0; u < CHUNK / 8; u++)
+				unpack_u16x8(w[u], kk[2 * u], kk[2 * u + 1]);
+		} else {
+			const int4 *__restrict__ v = reinterpret_cast<const int4 *>(row) + j0 / 4;
+#pragma unroll
+			for (int u = 0; u < CHUNK / 4; u++)
+				kk[u] = v[u];
+		}
 #pragma unroll
 		for (int u = 0; u < CHUNK / 4; u++) {
 			const int j = j0 + 4 * u;
@@ -335,9 +368,16 @@ __device__ __forceinline__ void ueg_wave_rows(const Tables &T, int m, int64_t id
 			f(j + 3, ml[j + 3], kk[u].w);
 		}
 	}
-	// rows are padded to a multiple of 4, so the last vector is in bounds
+	// rows are padded to 16 bytes, so the last 4-entry vector is in bounds
 	for (; j0 < nn; j0 += 4) {
-		const int4 k4 = row[j0 / 4];
+		int4 k4;
+		if (narrow) {
+			const uint2 w = reinterpret_cast<const uint2 *>(row)[j0 / 4];
+			k4 = make_int4((int)(w.x & 0xFFFFu), (int)(w.x >> 16), (int)(w.y & 0xFFFFu),
+			               (int)(w.y >> 16));
+		} else {
+			k4 = reinterpret_cast<const int4 *>(row)[j0 / 4];
+		}
 		f(j0, ml[j0], k4.x);
 		if (j0 + 1 < nn)
 			f(j0 + 1, ml[j0 + 1], k4.y);

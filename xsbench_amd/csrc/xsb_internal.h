@@ -47,7 +47,7 @@ struct Device {
 	// owned allocations
 	double *d_ueg = nullptr;
 	int32_t *d_index = nullptr;
-	int32_t *d_midx = nullptr;  // per-material unionized index tables
+	uint8_t *d_midx = nullptr;  // per-material unionized index tables (uint16 or int32)
 	double *d_grid = nullptr;
 	float *d_xs32 = nullptr;
 	uint16_t *d_xs16 = nullptr;

@@ -174,7 +174,7 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 	const float *__restrict__ ceff = mixed_consts<PREC>(T, mat);
 	const float *__restrict__ cbnd = ceff + T.max_num_nucs * kMaterials;
 	const int32_t *__restrict__ ml = T.mats + mat * T.max_num_nucs;
-	const int32_t *__restrict__ mrow = (GRID == kUnionized) ? ueg_row(T, mat, idx) : nullptr;
+	const uint8_t *__restrict__ mrow = (GRID == kUnionized) ? ueg_row(T, mat, idx) : nullptr;
 	float sum[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
 	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
 	float A = 0.f;
