@@ -253,6 +253,7 @@ def test_index_tables_width_boundary():
         with X.Device(inp, None) as a, X.Device(inp, None, idx32=True) as b:
             assert a.info()["bytes_midx"] == _midx_bytes(a, n_union, width)
             assert b.info()["bytes_midx"] == _midx_bytes(b, n_union, 4)
+            assert a.info()["bytes_ivl"] == n_union * 96  # one 12-double record per point
             for k in (X.K_BASELINE, X.K_MAT_E_SORT):
                 _, oa = a.event(1 << 16, 12345, kernel_id=k, detail=True)
                 _, ob = b.event(1 << 16, 12345, kernel_id=k, detail=True)

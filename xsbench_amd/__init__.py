@@ -155,7 +155,8 @@ class XsbInfo(C.Structure):
         ("device", C.c_int32), ("grid_type", C.c_int32), ("n_isotopes", C.c_int64),
         ("n_gridpoints", C.c_int64), ("n_union", C.c_int64), ("bytes_fp64", C.c_int64),
         ("bytes_fp32", C.c_int64), ("bytes_bf16", C.c_int64), ("bytes_index", C.c_int64),
-        ("bytes_midx", C.c_int64), ("bytes_union", C.c_int64), ("cu_count", C.c_int32),
+        ("bytes_midx", C.c_int64), ("bytes_union", C.c_int64), ("bytes_ivl", C.c_int64),
+        ("cu_count", C.c_int32),
         ("xcd_count", C.c_int32),
         ("arch", C.c_char * 32),
     ]

@@ -112,7 +112,7 @@ static void release(Device *d)
 	batch_state_free(d);
 	if (d->history_graph.exec)
 		(void)hipGraphExecDestroy(d->history_graph.exec);
-	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_midx, d->d_grid,  d->d_xs32, d->d_xs16,
+	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_midx, d->d_grid,  d->d_ivl, d->d_xs32, d->d_xs16,
 	                d->d_xs16_scale, d->d_num_nucs, d->d_mats, d->d_concs, d->d_concs32, d->d_acc,
 	                d->d_fftab};
 	for (void *p : ptrs)
@@ -416,19 +416,64 @@ static int download(Device *d, double *grid, double *ueg, int32_t *index)
 }
 
 // Material m's unionized index table: row e holds index_grid[e][mats[m][j]]
-// for j < nn, then zeros up to the pitch mp; IDX is the element type.
+// for j < nn, then zeros up to the pitch mp; IDX is the element type.  The
+// brackets are stored with the lookup's lower == n_gp - 1 -> n_gp - 2 rule
+// (Simulation.c:246-247) already applied.
 template <class IDX>
 __global__ void k_material_index(const int32_t *__restrict__ index, int64_t rows, int32_t pitch,
                                  const int32_t *__restrict__ ml, int32_t nn, int32_t mp,
-                                 IDX *__restrict__ dst)
+                                 int32_t n_gp, IDX *__restrict__ dst)
 {
 	const int64_t total = rows * mp;
 	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
 	     t += (int64_t)gridDim.x * blockDim.x) {
 		const int64_t e = t / mp;
 		const int j = (int)(t - e * mp);
-		dst[t] = (IDX)((j < nn) ? index[e * pitch + ml[j]] : 0);
+		int32_t v = (j < nn) ? index[e * pitch + ml[j]] : 0;
+		if (v == n_gp - 1)
+			v--;
+		dst[t] = (IDX)v;
 	}
+}
+
+// Interval records (Tables::ivl): record i = nuc * n_gp + k holds point k+1
+// and then point k+1 - point k, channel by channel; the last record of each
+// nuclide (k = n_gp - 1) is zero and never read.
+__global__ void k_interval_records(const double *__restrict__ grid, int64_t total, int32_t n_gp,
+                                   double *__restrict__ ivl)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const int64_t nuc = i / n_gp;
+		const bool last = (i - nuc * n_gp) == n_gp - 1;
+		const double *lo = grid + i * kPointDoubles;
+		double *r = ivl + i * kIvlDoubles;
+#pragma unroll
+		for (int c = 0; c < kPointDoubles; c++) {
+			const double hi = last ? 0.0 : lo[kPointDoubles + c];
+			r[c] = hi;
+			r[kPointDoubles + c] = last ? 0.0 : hi - lo[c];
+		}
+	}
+}
+
+static int build_interval_records(Device *d)
+{
+	Tables &T = d->T;
+	const int64_t total = (int64_t)T.n_iso * T.n_gp;
+	const size_t bytes = (size_t)total * kIvlDoubles * sizeof(double);
+	if (hipMalloc((void **)&d->d_ivl, bytes) != hipSuccess) {
+		(void)hipGetLastError();
+		d->d_ivl = nullptr;
+		return fail(XSB_ERR_OOM, "hipMalloc of %zu bytes (interval records) failed", bytes);
+	}
+	d->bytes_ivl = (int64_t)bytes;
+	hipLaunchKernelGGL(k_interval_records, dim3(4096), dim3(256), 0, d->stream, d->d_grid, total,
+	                   T.n_gp, d->d_ivl);
+	XSB_HIP(hipGetLastError());
+	XSB_HIP(hipStreamSynchronize(d->stream));
+	T.ivl = d->d_ivl;
+	return XSB_OK;
 }
 
 // Per-material copies of the unionized index grid (Tables::midx): each
@@ -462,11 +507,11 @@ static int build_material_index(Device *d, const xsb_tables *t, int64_t rows, in
 		if (narrow)
 			hipLaunchKernelGGL(k_material_index<uint16_t>, dim3(4096), dim3(256), 0, d->stream,
 			                   d->d_index, rows, (int32_t)pitch, ml, t->num_nucs[m],
-			                   T.midx_pitch[m], (uint16_t *)d->d_midx + T.midx_off[m]);
+			                   T.midx_pitch[m], T.n_gp, (uint16_t *)d->d_midx + T.midx_off[m]);
 		else
 			hipLaunchKernelGGL(k_material_index<int32_t>, dim3(4096), dim3(256), 0, d->stream,
 			                   d->d_index, rows, (int32_t)pitch, ml, t->num_nucs[m],
-			                   T.midx_pitch[m], (int32_t *)d->d_midx + T.midx_off[m]);
+			                   T.midx_pitch[m], T.n_gp, (int32_t *)d->d_midx + T.midx_off[m]);
 		XSB_HIP(hipGetLastError());
 	}
 	XSB_HIP(hipStreamSynchronize(d->stream));
@@ -602,6 +647,8 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 		d->num_nucs_host[m] = t->num_nucs[m];
 
 	if ((rc = build_compact(d, t, flags, xs_max)))
+		return rc;
+	if (t->grid_type == XSB_UNIONIZED && (rc = build_interval_records(d)))
 		return rc;
 	T.xs32 = reinterpret_cast<const float *>(d->d_xs32);
 	T.xs16 = d->d_xs16;
@@ -945,6 +992,7 @@ int xsb_hip_info(void *handle, xsb_info *info)
 	info->bytes_bf16 = d->bytes_bf16;
 	info->bytes_index = d->bytes_index;
 	info->bytes_midx = d->bytes_midx;
+	info->bytes_ivl = d->bytes_ivl;
 	info->bytes_union = d->bytes_union;
 	info->cu_count = d->cu_count;
 	info->xcd_count = d->xcd_count;

@@ -14,6 +14,7 @@ constexpr uint64_t kLcgA = 2806196910506780709ULL;
 constexpr uint64_t kLcgMask = 0x7FFFFFFFFFFFFFFFULL;  // mod 2^63
 constexpr int kMaterials = 12;
 constexpr int kPointDoubles = 6;  // energy + 5 XS channels (NuclideGridPoint)
+constexpr int kIvlDoubles = 12;   // interval record: upper point, then upper - lower
 
 enum GridType : int { kUnionized = 0, kNuclide = 1, kHash = 2 };
 
@@ -26,6 +27,9 @@ struct Tables {
 	const double *__restrict__ ueg;         // unionized energies [n_union]
 	const int32_t *__restrict__ index_grid; // [n_union or hash_bins][n_iso]
 	const double *__restrict__ grid;        // FP64 AoS, 6 doubles per point
+	// unionized grid: FP64 interval records for the wave path, record
+	// nuc * n_gp + k = point k+1 (E, 5 XS), then point k+1 - point k
+	const double *__restrict__ ivl;
 	const float *__restrict__ xs32;         // FP32 XS, 8 floats per point (5 used)
 	const uint16_t *__restrict__ xs16;      // BF16 XS, 8 halves per point (5 used)
 	const float *__restrict__ xs16_scale;   // per nuclide x channel scale (5 per nuclide, 8 stride)
@@ -246,6 +250,24 @@ __device__ __forceinline__ void accumulate_fp64(const double *__restrict__ pt, d
 	macro[4] += (f.y - fr * (f.y - c.y)) * conc;
 }
 
+// accumulate_fp64 from an interval record (Tables::ivl): the differences
+// hi - lo were rounded once when the record was built, as the reference
+// rounds them per lookup, so the result is the same.  Every operand but p
+// and the sums may come from the scalar cache: no copies into VGPRs.
+__device__ __forceinline__ void accumulate_ivl(const double *__restrict__ R, double p, double conc,
+                                               double macro[5])
+{
+	const double2 *q = reinterpret_cast<const double2 *>(R);
+	const double2 a = q[0], b = q[1], c = q[2];  // hi: E t | el ab | fi nf
+	const double2 d = q[3], e = q[4], f = q[5];  // hi - lo: E t | el ab | fi nf
+	const double fr = (a.x - p) / d.x;
+	macro[0] += (a.y - fr * d.y) * conc;
+	macro[1] += (b.x - fr * e.x) * conc;
+	macro[2] += (b.y - fr * e.y) * conc;
+	macro[3] += (c.x - fr * f.x) * conc;
+	macro[4] += (c.y - fr * f.y) * conc;
+}
+
 // calculate_macro_xs (openmp-threading/Simulation.c:326-375), FP64.
 template <int GRID>
 __device__ __forceinline__ void macro_xs_fp64(const Tables &T, double p, int mat, int64_t idx,
@@ -288,30 +310,67 @@ __device__ __forceinline__ void accumulate_fp64_select(const double *__restrict_
 		macro[c] += (hi[1 + c] - fr * (hi[1 + c] - lo[1 + c])) * conc;
 }
 
-// One nuclide of the wave-uniform unionized path, bracket k per lane.  The
-// points come through the scalar cache when the lanes' brackets coincide
-// (one pair for the wave), and with SHARE3 also when they span two
-// neighbouring intervals (three points, each lane picks its pair); else per
-// lane.  The scheduling barrier after a three-point step keeps the compiler
-// from overlapping several of them (which held ~200 VGPRs live across the
-// unrolled chunk: two waves per SIMD instead of six).
-template <bool SHARE3>
+// Interval records k0 and k0+1 of one nuclide are shared by the wave (scalar
+// loads); each lane interpolates in record sel (0 or 1).
+__device__ __forceinline__ void accumulate_ivl_select(const double *__restrict__ R, uint32_t sel,
+                                                      double p, double conc, double macro[5])
+{
+	double a[2 * kIvlDoubles];
+#pragma unroll
+	for (int w = 0; w < 2 * kIvlDoubles; w++)
+		a[w] = R[w];
+	double r[kIvlDoubles];
+#pragma unroll
+	for (int w = 0; w < kIvlDoubles; w++)
+		r[w] = sel ? a[kIvlDoubles + w] : a[w];
+	const double fr = (r[0] - p) / r[6];
+#pragma unroll
+	for (int c = 0; c < 5; c++)
+		macro[c] += (r[1 + c] - fr * r[7 + c]) * conc;
+}
+
+// One nuclide of the wave-uniform paths, bracket k per lane.  The grid data
+// come through the scalar cache when the lanes' brackets coincide (one
+// interval for the wave), and with SHARE3 also when they span two
+// neighbouring intervals (each lane picks its own); else per lane.  IVL
+// reads the interval records (unionized grid: brackets stored with the
+// lower == n_gp - 1 rule applied), else the points (hash grid: the rule is
+// applied here).  The scheduling barrier after a two-interval step keeps the
+// compiler from overlapping several of them (which held ~200 VGPRs live
+// across the unrolled chunk: two waves per SIMD instead of eight).
+template <bool SHARE3, bool IVL>
 __device__ __forceinline__ void ueg_wave_nuclide(const Tables &T, int nuc, int k, double p,
                                                  double conc, double macro[5])
 {
 	const int32_t n_gp = T.n_gp;
-	if (k == n_gp - 1)
+	if (!IVL && k == n_gp - 1)
 		k--;
 	const int k0 = __builtin_amdgcn_readfirstlane(k);
-	const int d = k - k0;
-	const double *P = T.grid + ((int64_t)nuc * n_gp + k0) * kPointDoubles;
-	if (__ballot(d != 0) == 0ULL) {
+	const uint32_t d = (uint32_t)(k - k0);
+	const int64_t i0 = (int64_t)nuc * n_gp + k0;
+	if (IVL) {
+		const double *R = T.ivl + i0 * kIvlDoubles;
+		if (__ballot(d != 0u) == 0ULL) {
+			accumulate_ivl(R, p, conc, macro);
+		} else if (SHARE3 && __ballot(d > 1u) == 0ULL) {
+			accumulate_ivl_select(R, d, p, conc, macro);
+			__builtin_amdgcn_sched_barrier(0);
+		} else if (SHARE3 && __ballot(d + 1u > 1u) == 0ULL) {
+			accumulate_ivl_select(R - kIvlDoubles, d + 1u, p, conc, macro);
+			__builtin_amdgcn_sched_barrier(0);
+		} else {
+			accumulate_ivl(T.ivl + ((int64_t)nuc * n_gp + k) * kIvlDoubles, p, conc, macro);
+		}
+		return;
+	}
+	const double *P = T.grid + i0 * kPointDoubles;
+	if (__ballot(d != 0u) == 0ULL) {
 		accumulate_fp64(P, p, conc, macro);
-	} else if (SHARE3 && __ballot(d != 0 && d != 1) == 0ULL) {
-		accumulate_fp64_select(P, d, p, conc, macro);
+	} else if (SHARE3 && __ballot(d > 1u) == 0ULL) {
+		accumulate_fp64_select(P, (int)d, p, conc, macro);
 		__builtin_amdgcn_sched_barrier(0);
-	} else if (SHARE3 && __ballot(d != 0 && d != -1) == 0ULL) {
-		accumulate_fp64_select(P - kPointDoubles, d + 1, p, conc, macro);
+	} else if (SHARE3 && __ballot(d + 1u > 1u) == 0ULL) {
+		accumulate_fp64_select(P - kPointDoubles, (int)(d + 1u), p, conc, macro);
 		__builtin_amdgcn_sched_barrier(0);
 	} else {
 		accumulate_fp64(T.grid + ((int64_t)nuc * n_gp + k) * kPointDoubles, p, conc, macro);
@@ -401,7 +460,7 @@ __device__ __forceinline__ void macro_ueg_wave(const Tables &T, double p, int m,
 		macro[c] = 0.0;
 	const double *__restrict__ cl = T.concs + m * T.max_num_nucs;
 	ueg_wave_rows<CHUNK>(T, m, idx, [&](int j, int nuc, int k) {
-		ueg_wave_nuclide<SHARE3>(T, nuc, k, p, cl[j], macro);
+		ueg_wave_nuclide<SHARE3, true>(T, nuc, k, p, cl[j], macro);
 	});
 }
 
@@ -457,7 +516,7 @@ __device__ __forceinline__ void macro_hash_wave(const Tables &T, double p, int m
 		macro[c] = 0.0;
 	const double *__restrict__ cl = T.concs + m * T.max_num_nucs;
 	hash_wave_rows<kPointDoubles>(T, T.grid, p, m, idx, [&](int j, int nuc, int k) {
-		ueg_wave_nuclide<SHARE3>(T, nuc, k, p, cl[j], macro);
+		ueg_wave_nuclide<SHARE3, false>(T, nuc, k, p, cl[j], macro);
 	});
 }
 

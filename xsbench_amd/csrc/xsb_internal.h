@@ -49,6 +49,7 @@ struct Device {
 	int32_t *d_index = nullptr;
 	uint8_t *d_midx = nullptr;  // per-material unionized index tables (uint16 or int32)
 	double *d_grid = nullptr;
+	double *d_ivl = nullptr;  // FP64 interval records (Tables::ivl)
 	float *d_xs32 = nullptr;
 	uint16_t *d_xs16 = nullptr;
 	float *d_xs16_scale = nullptr;
@@ -59,7 +60,7 @@ struct Device {
 	unsigned long long *d_acc = nullptr;  // 64 accumulators
 	uint64_t *d_fftab = nullptr;          // LCG skip-ahead table
 	int64_t bytes_fp64 = 0, bytes_fp32 = 0, bytes_bf16 = 0, bytes_index = 0, bytes_union = 0;
-	int64_t bytes_midx = 0;
+	int64_t bytes_midx = 0, bytes_ivl = 0;
 	int32_t num_nucs_host[kMaterials] = {0};
 	Scratch scratch[4];
 	hipEvent_t ev[16] = {};

@@ -150,7 +150,7 @@ SimulationData move_simulation_data_to_device(Inputs in, int mype, SimulationDat
 		if (xsb_hip_info(h, &info) == XSB_OK)
 			printf("GPU %s: %d CUs, %.0lf MB of tables resident\n", info.arch, info.cu_count,
 			       (info.bytes_fp64 + info.bytes_fp32 + info.bytes_bf16 + info.bytes_index +
-			        info.bytes_midx + info.bytes_union) / 1048576.0);
+			        info.bytes_midx + info.bytes_union + info.bytes_ivl) / 1048576.0);
 	}
 	return SD;
 }
