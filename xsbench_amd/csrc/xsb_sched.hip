@@ -20,10 +20,9 @@ constexpr unsigned kXcdRun = 16;  // lookup blocks per XCD run
 
 enum SortMode : int { kModeMat = 0, kModeFuel = 1, kModeMatE = 2 };
 
-__device__ __forceinline__ uint32_t sample_key(const Tables &T, uint64_t seed0, int64_t lookup,
-                                               int mode)
+// Sort key of the lookup whose LCG state (before its two draws) is s.
+__device__ __forceinline__ uint32_t sample_key(const Tables &T, uint64_t s, int mode)
 {
-	uint64_t s = ff_jump(seed0, 2ULL * (uint64_t)lookup, T.ff_tab);
 	const double p = lcg_double(s);
 	const int mat = pick_mat(s, T.thr);
 	if (mode == kModeFuel)
@@ -47,14 +46,25 @@ __global__ __launch_bounds__(kSortBlock) void k_keys(Tables T, int64_t first, in
 	__syncthreads();
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int64_t base = (int64_t)blockIdx.x * kTile;
+	// A wave's lanes hold consecutive lookups (tile_pos), two draws each:
+	// the wave's first state s0 is one full skip-ahead, a lane's state is s0
+	// advanced 2 * lane draws and the next item's s0 is 128 draws on, both
+	// single radix-256 digits of the skip table (entry 0 is the identity).
+	const int64_t e0 = base + tile_pos(wave, 0, 0);
+	uint64_t s0 = ff_jump(seed0, 2ULL * (uint64_t)(first + e0), T.ff_tab);
+	const ulonglong2 lane_skip = reinterpret_cast<const ulonglong2 *>(T.ff_tab)[2 * lane];
+	const ulonglong2 item_skip = reinterpret_cast<const ulonglong2 *>(T.ff_tab)[128];
+	static_assert(kSortItems > 1 && 2 * 63 < 256, "skips within one digit");
 	for (int it = 0; it < kSortItems; it++) {
 		const int64_t e = base + tile_pos(wave, it, lane);
 		if (e < n) {
-			const uint32_t k = sample_key(T, seed0, first + e, mode);
+			const uint64_t s = (lane_skip.x * s0 + lane_skip.y) & kLcgMask;
+			const uint32_t k = sample_key(T, s, mode);
 			keys[e] = k;
 			vals[e] = (uint32_t)e;
 			atomicAdd(&h[k & kDigitMask], 1u);
 		}
+		s0 = (item_skip.x * s0 + item_skip.y) & kLcgMask;
 	}
 	__syncthreads();
 	write_tile_hist(h, hist, ntiles);
@@ -70,11 +80,16 @@ __global__ __launch_bounds__(kSortBlock) void k_hist(const K *__restrict__ keys,
 	__syncthreads();
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int64_t base = (int64_t)blockIdx.x * kTile;
+	uint32_t dg[kSortItems];  // all loads in flight before the counting
+#pragma unroll
 	for (int it = 0; it < kSortItems; it++) {
 		const int64_t e = base + tile_pos(wave, it, lane);
-		if (e < n)
-			atomicAdd(&h[(uint32_t)(keys[e] >> shift) & kDigitMask], 1u);
+		dg[it] = e < n ? (uint32_t)(keys[e] >> shift) & kDigitMask : (uint32_t)kBins;
 	}
+#pragma unroll
+	for (int it = 0; it < kSortItems; it++)
+		if (dg[it] < (uint32_t)kBins)
+			atomicAdd(&h[dg[it]], 1u);
 	__syncthreads();
 	write_tile_hist(h, hist, ntiles);
 }
@@ -89,6 +104,25 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane)
 			x += y;
 	}
 	return x;
+}
+
+// exclusive scan of one value per thread over a BLOCK-thread block
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_excl_scan_n(uint32_t x)
+{
+	constexpr int kW = BLOCK / 64;
+	__shared__ uint32_t wsum[kW];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const uint32_t incl = wave_incl_scan(x, lane);
+	if (lane == 63)
+		wsum[wave] = incl;
+	__syncthreads();
+	uint32_t off = 0;
+#pragma unroll
+	for (int w = 0; w < kW; w++)
+		off += (w < wave) ? wsum[w] : 0u;
+	__syncthreads();
+	return off + incl - x;
 }
 
 // exclusive scan of one value per thread over a 256-thread block
@@ -241,7 +275,11 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ ki
 // out in runs, consecutive lanes storing consecutive addresses.  After the
 // last pass the keys are sorted; elements with equal keys may be permuted
 // within a tile.
-__global__ __launch_bounds__(kSortBlock) void k_scatter_runs(const uint32_t *__restrict__ kin,
+constexpr int kRunsBlock = 1024;                 // 16 waves: two blocks fill a CU
+constexpr int kRunsItems = kTile / kRunsBlock;   // 8 elements per thread
+static_assert(kBins == kRunsBlock, "one digit per thread in the local scan");
+
+__global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint32_t *__restrict__ kin,
                                                              const uint32_t *__restrict__ vin,
                                                              uint32_t *__restrict__ kout,
                                                              uint32_t *__restrict__ vout,
@@ -252,42 +290,35 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter_runs(const uint32_t *__r
 	__shared__ uint32_t cnt[kBins];   // digit counts, then the tile's local starts
 	__shared__ uint32_t gbase[kBins]; // global start minus local start, per digit
 	__shared__ uint32_t sk[kTile], sv[kTile];
-	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
-		cnt[d] = 0;
+	cnt[threadIdx.x] = 0;
 	__syncthreads();
 	const int64_t base = (int64_t)blockIdx.x * kTile;
 	const int m = (int)min((int64_t)kTile, n - base);
-	uint32_t kk[kSortItems], vv[kSortItems], rank[kSortItems];
+	uint32_t kk[kRunsItems], vv[kRunsItems], rank[kRunsItems];
 #pragma unroll
-	for (int it = 0; it < kSortItems; it++) {
-		const int i = it * kSortBlock + threadIdx.x;
+	for (int it = 0; it < kRunsItems; it++) {
+		const int i = it * kRunsBlock + threadIdx.x;
 		if (i < m) {
 			kk[it] = kin[base + i];
 			vv[it] = vin[base + i];
-			rank[it] = atomicAdd(&cnt[(kk[it] >> shift) & kDigitMask], 1u);
 		}
 	}
-	__syncthreads();
-	// local exclusive scan: thread t owns digits 4t .. 4t+3
-	static_assert(kBins == 4 * kSortBlock, "four digits per thread");
-	uint32_t c[4], sum = 0;
 #pragma unroll
-	for (int j = 0; j < 4; j++) {
-		c[j] = cnt[4 * threadIdx.x + j];
-		sum += c[j];
-	}
-	uint32_t run = block_excl_scan(sum, nullptr);
-#pragma unroll
-	for (int j = 0; j < 4; j++) {
-		const int d = 4 * threadIdx.x + j;
-		cnt[d] = run;
-		gbase[d] = hist[(int64_t)d * ntiles + blockIdx.x] - run;
-		run += c[j];
+	for (int it = 0; it < kRunsItems; it++) {
+		const int i = it * kRunsBlock + threadIdx.x;
+		if (i < m)
+			rank[it] = atomicAdd(&cnt[(kk[it] >> shift) & kDigitMask], 1u);
 	}
 	__syncthreads();
+	const int d = threadIdx.x;
+	const uint32_t c = cnt[d];
+	const uint32_t run = block_excl_scan_n<kRunsBlock>(c);
+	cnt[d] = run;
+	gbase[d] = hist[(int64_t)d * ntiles + blockIdx.x] - run;
+	__syncthreads();
 #pragma unroll
-	for (int it = 0; it < kSortItems; it++) {
-		const int i = it * kSortBlock + threadIdx.x;
+	for (int it = 0; it < kRunsItems; it++) {
+		const int i = it * kRunsBlock + threadIdx.x;
 		if (i < m) {
 			const uint32_t pos = cnt[(kk[it] >> shift) & kDigitMask] + rank[it];
 			sk[pos] = kk[it];
@@ -295,11 +326,15 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter_runs(const uint32_t *__r
 		}
 	}
 	__syncthreads();
-	for (int i = threadIdx.x; i < m; i += kSortBlock) {
-		const uint32_t key = sk[i];
-		const uint32_t g = gbase[(key >> shift) & kDigitMask] + (uint32_t)i;
-		kout[g] = key;
-		vout[g] = sv[i];
+#pragma unroll
+	for (int it = 0; it < kRunsItems; it++) {
+		const int i = it * kRunsBlock + threadIdx.x;
+		if (i < m) {
+			const uint32_t key = sk[i];
+			const uint32_t g = gbase[(key >> shift) & kDigitMask] + (uint32_t)i;
+			kout[g] = key;
+			vout[g] = sv[i];
+		}
 	}
 }
 
@@ -439,7 +474,7 @@ int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint32_t
 		}
 		if ((rc = scan_hist(hist, hlen, part, s)))
 			return rc;
-		hipLaunchKernelGGL(k_scatter_runs, dim3(ntiles), dim3(kSortBlock), 0, s, *k, *v, *k2, *v2,
+		hipLaunchKernelGGL(k_scatter_runs, dim3(ntiles), dim3(kRunsBlock), 0, s, *k, *v, *k2, *v2,
 		                   n, shift, hist, ntiles);
 		XSB_HIP(hipGetLastError());
 		uint32_t *t = *k;
