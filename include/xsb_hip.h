@@ -143,7 +143,7 @@ typedef struct {
 	int32_t graph;     /* 1 if the pipeline ran as a hipGraph */
 	int32_t mapping;   /* 1: one lane per lookup; 2: wave-uniform path where a wave's
 	                      lanes share the material (and on the hash grid the bin);
-	                      3: as 2 without the three-point shared step */
+	                      3: as 2 without the two-interval shared step */
 	int32_t reserved0;
 	double ms_tuning;  /* host wall time spent tuning during this call */
 } xsb_batch_plan;
@@ -163,7 +163,7 @@ typedef struct {
 	int64_t window;            /* mode 0: lookups per sort window (0 = one window) */
 	int32_t eager;             /* mode 0: 1 = launch the kernels directly instead of the graph */
 	int32_t mapping;           /* mode 0: 1 lane per lookup, 2 wave-uniform, 3 wave-uniform
-	                              without the three-point step (0 = autotuned) */
+	                              without the two-interval step (0 = autotuned) */
 	uint64_t *out_vsum;        /* optional host [n_batches] */
 	double *out_done_ms;       /* optional host [n_batches]: completion time of each batch,
 	                              from the start of the call's first kernel */

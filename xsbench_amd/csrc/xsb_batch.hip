@@ -453,7 +453,7 @@ static int launch_or_size(Device *d, const BatchView &B, const SortedQueue &q, b
 	return XSB_OK;
 }
 
-// dispatch on (grid type, precision, staging depth, three-point step); the
+// dispatch on (grid type, precision, staging depth, two-interval step); the
 // staging depth only matters on the unionized grid
 template <int GRID, int PREC>
 static int dispatch_bsorted(Device *d, const BatchView &B, const SortedQueue &q, int stage,
@@ -499,7 +499,7 @@ static int run_bsorted(Device *d, int prec, const BatchView &B, const SortedQueu
 // ---- host side of mode 0 ----
 // mapping: 1 one lane per lookup (the reference loop per lane), 2 the
 // wave-uniform path where a wave's lanes share the material (and, on the
-// hash grid, the bin), 3 the same without the three-point shared step
+// hash grid, the bin), 3 the same without the two-interval shared step
 struct BatchPlan {
 	int tile = 0, stage = 0, sort = 0, mapping = 0;
 	bool fits(const BatchPlan &fixed) const

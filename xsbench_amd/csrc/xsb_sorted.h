@@ -59,7 +59,7 @@ __device__ __forceinline__ void write_tile_hist(const uint32_t *h, uint32_t *his
 // One lane's lookup (p, mat).  Unionized waves whose lanes share the
 // material take the wave-uniform path (macro_ueg_wave / _mixed, index words
 // staged CHUNK deep) when `wave` is set; everything else runs the per-lane
-// loop.  SHARE3 selects the wave paths' three-point step (ueg_wave_nuclide).  Returns the argmax channel; idx gets the grid-level index,
+// loop.  SHARE3 selects the wave paths' two-interval step (ueg_wave_nuclide).  Returns the argmax channel; idx gets the grid-level index,
 // macro the sums, fb the fallback flag (mixed precision).
 // HISTORY: the caller also tests macro_c > 1.0, so mixed precision falls
 // back to FP64 near 1.0 too and then returns the FP64 sums.
