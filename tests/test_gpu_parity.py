@@ -205,6 +205,23 @@ def test_device_grid_construction_tiny_grids():
         assert np.array_equal(got["index_grid"].reshape(-1), want["index_grid"]), (grid, gp)
 
 
+@pytest.mark.parametrize("grid,gp", [("unionized", 2), ("unionized", 3), ("hash", 3),
+                                     ("unionized", 40)])
+def test_tiny_grid_lookups_match_oracle(orc, grid, gp):
+    """lookups on 2-, 3- and 40-point grids (every bracket at the clamped
+    last interval, records next to the zero record) on the sorted and the
+    fused schedules, against the oracle"""
+    inp = X.make_inputs("small", grid, "event", gridpoints=gp, hash_bins=17)
+    sd = X.grid_init(inp)
+    oi, osd = as_oracle(inp, sd)
+    first, n = 1000, 1 << 14
+    E, m, idx, macro, am = orc.event_detail(oi, osd, np.arange(first, first + n))
+    with X.Device(inp, sd) as dev:
+        for k in (X.K_BASELINE, X.K_MAT_E_SORT):
+            _, out = dev.event(n, first, kernel_id=k, detail=True)
+            assert same_bits(out["macro"], macro) and np.array_equal(out["argmax"], am), (grid, gp, k)
+
+
 # ---------------------------------------------------------------------------
 # per-material unionized index tables: uint16 brackets when n_gridpoints <=
 # 65536 (the default), int32 otherwise or on request (BUILD_IDX32)

@@ -112,7 +112,7 @@ static void release(Device *d)
 	batch_state_free(d);
 	if (d->history_graph.exec)
 		(void)hipGraphExecDestroy(d->history_graph.exec);
-	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_midx, d->d_grid,  d->d_ivl, d->d_xs32, d->d_xs16,
+	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_midx, d->d_grid,  d->d_ivl, d->d_ivl32, d->d_xs32, d->d_xs16,
 	                d->d_xs16_scale, d->d_num_nucs, d->d_mats, d->d_concs, d->d_concs32, d->d_acc,
 	                d->d_fftab};
 	for (void *p : ptrs)
@@ -457,6 +457,33 @@ __global__ void k_interval_records(const double *__restrict__ grid, int64_t tota
 	}
 }
 
+// FP32 interval records (Tables::ivl32): {E_{k+1}, E_{k+1} - E_k,
+// float(xs_{k+1})[5], float(xs_{k+1}) - float(xs_k) [5], pad}; zero for the last
+// point of each nuclide.
+__global__ void k_interval_records32(const double *__restrict__ grid, int64_t total, int32_t n_gp,
+                                     double *__restrict__ ivl)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const int64_t nuc = i / n_gp;
+		const bool last = (i - nuc * n_gp) == n_gp - 1;
+		const double *lo = grid + i * kPointDoubles;
+		double *r = ivl + i * kIvl32Doubles;
+		float *f = reinterpret_cast<float *>(r + 2);
+		r[0] = last ? 0.0 : lo[kPointDoubles];
+		r[1] = last ? 0.0 : lo[kPointDoubles] - lo[0];
+#pragma unroll
+		for (int c = 0; c < 5; c++) {
+			const float xh = last ? 0.f : (float)lo[kPointDoubles + 1 + c];
+			const float xl = (float)lo[1 + c];
+			f[c] = xh;
+			f[5 + c] = last ? 0.f : xh - xl;
+		}
+		f[10] = 0.f;
+		f[11] = 0.f;
+	}
+}
+
 static int build_interval_records(Device *d)
 {
 	Tables &T = d->T;
@@ -471,8 +498,22 @@ static int build_interval_records(Device *d)
 	hipLaunchKernelGGL(k_interval_records, dim3(4096), dim3(256), 0, d->stream, d->d_grid, total,
 	                   T.n_gp, d->d_ivl);
 	XSB_HIP(hipGetLastError());
+	if (d->d_xs32) {
+		const size_t bytes32 = (size_t)total * kIvl32Doubles * sizeof(double);
+		if (hipMalloc((void **)&d->d_ivl32, bytes32) != hipSuccess) {
+			(void)hipGetLastError();
+			d->d_ivl32 = nullptr;
+			return fail(XSB_ERR_OOM, "hipMalloc of %zu bytes (FP32 interval records) failed",
+			            bytes32);
+		}
+		d->bytes_ivl += (int64_t)bytes32;
+		hipLaunchKernelGGL(k_interval_records32, dim3(4096), dim3(256), 0, d->stream, d->d_grid,
+		                   total, T.n_gp, d->d_ivl32);
+		XSB_HIP(hipGetLastError());
+	}
 	XSB_HIP(hipStreamSynchronize(d->stream));
 	T.ivl = d->d_ivl;
+	T.ivl32 = d->d_ivl32;
 	return XSB_OK;
 }
 

@@ -30,6 +30,9 @@ struct Tables {
 	// unionized grid: FP64 interval records for the wave path, record
 	// nuc * n_gp + k = point k+1 (E, 5 XS), then point k+1 - point k
 	const double *__restrict__ ivl;
+	// unionized grid with FP32 tables: FP32 interval records, 64 B each:
+	// {E_{k+1}, E_{k+1} - E_k, float xs_{k+1}[5], float xs_{k+1} - xs_k [5]}
+	const double *__restrict__ ivl32;
 	const float *__restrict__ xs32;         // FP32 XS, 8 floats per point (5 used)
 	const uint16_t *__restrict__ xs16;      // BF16 XS, 8 halves per point (5 used)
 	const float *__restrict__ xs16_scale;   // per nuclide x channel scale (5 per nuclide, 8 stride)

@@ -49,7 +49,8 @@ struct Device {
 	int32_t *d_index = nullptr;
 	uint8_t *d_midx = nullptr;  // per-material unionized index tables (uint16 or int32)
 	double *d_grid = nullptr;
-	double *d_ivl = nullptr;  // FP64 interval records (Tables::ivl)
+	double *d_ivl = nullptr;    // FP64 interval records (Tables::ivl)
+	double *d_ivl32 = nullptr;  // FP32 interval records (Tables::ivl32)
 	float *d_xs32 = nullptr;
 	uint16_t *d_xs16 = nullptr;
 	float *d_xs16_scale = nullptr;

@@ -35,6 +35,7 @@ namespace xsb {
 constexpr int kRec32Doubles = 4;  // 32 B record
 constexpr double kRelTol = 1e-5;  // FP32 tables: certified relative error per channel
 constexpr int kRec16Doubles = 3;  // 24 B record
+constexpr int kIvl32Doubles = 8;  // 64 B FP32 interval record (Tables::ivl32)
 
 // Per (material, position) constants for the mixed kernels, [12][max_num_nucs]
 // float2 {c_eff, c_bound}: c_eff = conc * scale (scale = 1 for FP32),
@@ -98,6 +99,23 @@ __device__ __forceinline__ void mixed_term(double e_lo, const float xl[5], doubl
 #pragma unroll
 	for (int k = 0; k < 5; k++) {
 		const float x = __builtin_fmaf(-fr, xh[k] - xl[k], xh[k]);
+		kahan_add(sum[k], comp[k], x * c);
+	}
+	A += cb * (1.0f + 2.0f * fabsf(fr));
+}
+
+// mixed_term from an FP32 interval record: the same operations on the same
+// values (the FP64 width and the FP32 differences were rounded once when the
+// record was built, exactly as mixed_term rounds them).
+__device__ __forceinline__ void mixed_term_ivl(const double *__restrict__ R, double p, float c,
+                                               float cb, float sum[5], float comp[5], float &A)
+{
+	const double fr64 = (R[0] - p) / R[1];
+	const float fr = (float)fr64;
+	const float *f = reinterpret_cast<const float *>(R + 2);  // xh[5], d[5]
+#pragma unroll
+	for (int k = 0; k < 5; k++) {
+		const float x = __builtin_fmaf(-fr, f[5 + k], f[k]);
 		kahan_add(sum[k], comp[k], x * c);
 	}
 	A += cb * (1.0f + 2.0f * fabsf(fr));
@@ -198,7 +216,9 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 // lane: records through the scalar cache when the lanes' brackets coincide,
 // and with SHARE3 also when they span two neighbouring intervals, else per
 // lane (as ueg_wave_nuclide).
-template <int PREC, bool SHARE3 = true>
+// With IVL (unionized grid, FP32 tables) the FP32 interval records are read
+// instead, as ueg_wave_nuclide does in FP64.
+template <int PREC, bool SHARE3 = true, bool IVL = false>
 __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc, int k, double p,
                                                        float c, float cb, float sum[5],
                                                        float comp[5], float &A)
@@ -209,6 +229,31 @@ __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc,
 		k--;
 	const int k0 = __builtin_amdgcn_readfirstlane(k);
 	const int d = k - k0;
+	if (IVL && PREC == XSB_XS_FP32) {
+		const uint32_t du = (uint32_t)d;
+		const double *R = T.ivl32 + ((int64_t)nuc * n_gp + k0) * kIvl32Doubles;
+		if (__ballot(du != 0u) == 0ULL) {
+			mixed_term_ivl(R, p, c, cb, sum, comp, A);
+		} else if (SHARE3 && (__ballot(du > 1u) == 0ULL || __ballot(du + 1u > 1u) == 0ULL)) {
+			// two records shared; each lane takes its own
+			const bool back = __ballot(du > 1u) != 0ULL;
+			const double *Q = back ? R - kIvl32Doubles : R;
+			const uint32_t sel = back ? du + 1u : du;
+			double a[2 * kIvl32Doubles], r[kIvl32Doubles];
+#pragma unroll
+			for (int w = 0; w < 2 * kIvl32Doubles; w++)
+				a[w] = Q[w];
+#pragma unroll
+			for (int w = 0; w < kIvl32Doubles; w++)
+				r[w] = sel ? a[kIvl32Doubles + w] : a[w];
+			mixed_term_ivl(r, p, c, cb, sum, comp, A);
+			__builtin_amdgcn_sched_barrier(0);
+		} else {
+			mixed_term_ivl(T.ivl32 + ((int64_t)nuc * n_gp + k) * kIvl32Doubles, p, c, cb, sum,
+			               comp, A);
+		}
+		return;
+	}
 	const double *row = mixed_row<PREC>(T, nuc);
 	const double *P = row + (int64_t)k0 * REC;
 	float x0[5], x1[5];
@@ -255,7 +300,7 @@ __device__ __forceinline__ int macro_ueg_wave_mixed(const Tables &T, double p, i
 	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
 	float A = 0.f;
 	ueg_wave_rows<CHUNK>(T, m, idx, [&](int j, int nuc, int k) {
-		ueg_wave_nuclide_mixed<PREC, SHARE3>(T, nuc, k, p, ceff[j], cbnd[j], sum, comp, A);
+		ueg_wave_nuclide_mixed<PREC, SHARE3, true>(T, nuc, k, p, ceff[j], cbnd[j], sum, comp, A);
 	});
 	return mixed_finish<PREC, HISTORY, true>(sum, A, macro, fell_back, bound, [&](double m64[5]) {
 		macro_ueg_wave<CHUNK, SHARE3>(T, p, m, idx, m64);
