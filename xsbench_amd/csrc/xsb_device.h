@@ -314,22 +314,26 @@ __device__ __forceinline__ void accumulate_fp64_select(const double *__restrict_
 }
 
 // Interval records k0 and k0+1 of one nuclide are shared by the wave (scalar
-// loads); each lane interpolates in record sel (0 or 1).
+// loads); each lane needs record sel (0 or 1).  Both interpolations run on
+// every lane with the records' values straight from SGPRs and each lane keeps
+// its own: gfx950 reads one SGPR per VALU instruction, so selecting the
+// records' 24 dwords per lane would cost three instructions each, more than
+// the second interpolation.  The kept sums are those of the lane's record
+// alone, bit for bit.
 __device__ __forceinline__ void accumulate_ivl_select(const double *__restrict__ R, uint32_t sel,
                                                       double p, double conc, double macro[5])
 {
-	double a[2 * kIvlDoubles];
+	double a[5], b[5];
 #pragma unroll
-	for (int w = 0; w < 2 * kIvlDoubles; w++)
-		a[w] = R[w];
-	double r[kIvlDoubles];
-#pragma unroll
-	for (int w = 0; w < kIvlDoubles; w++)
-		r[w] = sel ? a[kIvlDoubles + w] : a[w];
-	const double fr = (r[0] - p) / r[6];
+	for (int c = 0; c < 5; c++) {
+		a[c] = macro[c];
+		b[c] = macro[c];
+	}
+	accumulate_ivl(R, p, conc, a);
+	accumulate_ivl(R + kIvlDoubles, p, conc, b);
 #pragma unroll
 	for (int c = 0; c < 5; c++)
-		macro[c] += (r[1 + c] - fr * r[7 + c]) * conc;
+		macro[c] = sel ? b[c] : a[c];
 }
 
 // One nuclide of the wave-uniform paths, bracket k per lane.  The grid data
