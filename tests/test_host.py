@@ -201,6 +201,49 @@ def test_hip_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
 
 
+_ABI_STRUCTS = {"xsb_tables": "XsbTables", "xsb_event_args": "XsbEventArgs",
+                "xsb_history_args": "XsbHistoryArgs", "xsb_batch_plan": "XsbBatchPlan",
+                "xsb_batched_args": "XsbBatchedArgs", "xsb_result": "XsbResult",
+                "xsb_info": "XsbInfo"}
+_ABI_CONSTS = {"XSB_XS_FP64": "XS_FP64", "XSB_XS_FP32": "XS_FP32", "XSB_XS_BF16": "XS_BF16",
+               "XSB_BUILD_FP32": "BUILD_FP32", "XSB_BUILD_BF16": "BUILD_BF16",
+               "XSB_BUILD_GRIDS": "BUILD_GRIDS", "XSB_BUILD_IDX32": "BUILD_IDX32",
+               "XSB_UNIONIZED": "UNIONIZED", "XSB_NUCLIDE": "NUCLIDE", "XSB_HASH": "HASH",
+               "XSB_K_BASELINE": "K_BASELINE", "XSB_K_MAT_E_SORT": "K_MAT_E_SORT",
+               "XSB_K_AUTO": "K_AUTO"}
+
+
+def test_abi_structs_and_constants_match_the_python_mirror(tmp_path):
+    """sizes and field offsets of every xsb_hip.h struct, and its constants,
+    as the C compiler sees them, against the ctypes mirror"""
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "xsb_hip.h"', "int main(void) {"]
+    for cs, ps in _ABI_STRUCTS.items():
+        lines.append(f'printf("{cs} size %zu\\n", sizeof({cs}));')
+        for f, _ in getattr(X, ps)._fields_:
+            lines.append(f'printf("{cs} {f} %zu\\n", offsetof({cs}, {f}));')
+    for c in _ABI_CONSTS:
+        lines.append(f'printf("const {c} %lld\\n", (long long)({c}));')
+    lines.append("return 0; }")
+    src = tmp_path / "abi.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "abi"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    seen = 0
+    for line in filter(None, out):
+        a, b, v = line.split()
+        if a == "const":
+            assert int(v) == getattr(X, _ABI_CONSTS[b]), line
+        elif b == "size":
+            assert int(v) == C.sizeof(getattr(X, _ABI_STRUCTS[a])), line
+        else:
+            assert int(v) == getattr(getattr(X, _ABI_STRUCTS[a]), b).offset, line
+        seen += 1
+    assert seen == len(_ABI_CONSTS) + sum(1 + len(getattr(X, p)._fields_)
+                                          for p in _ABI_STRUCTS.values())
+
+
 def test_host_library_exports_every_declared_function():
     text = open(os.path.join(ROOT, "include", "xsbench.h")).read()
     names = set(re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", text, re.M))
