@@ -44,10 +44,21 @@ def traffic_from_profile(kernel_id):
     return vals.get(key)
 
 
-def cpu_baseline(inp, sd):
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(inp, sd, runs=3):
     """The reference's own openmp-threading event loop (oracle/_ref build) on
     the same tables, or this project's C restatement where that build is
-    absent; all 17M lookups of the workload, one run."""
+    absent; all 17M lookups of the workload, median of `runs` runs, timed
+    around the run_* call as the reference times it (sampling included)."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     gomp = C.CDLL("libgomp.so.1")
@@ -59,21 +70,28 @@ def cpu_baseline(inp, sd):
     try:
         ref = oracle.Reference()
         kind = "reference"
-        t0 = time.perf_counter()
-        v = ref.run_event_based_simulation(oi, osd, 1)
-        dt = time.perf_counter() - t0
+
+        def run():
+            return ref.run_event_based_simulation(oi, osd, 1)
     except (FileNotFoundError, OSError):
         orc = oracle.Oracle()
         kind = "port"
+
+        def run():
+            return orc.run_event(C.byref(oi), C.byref(osd), threads)
+    times = []
+    for _ in range(runs):
         t0 = time.perf_counter()
-        v = orc.run_event(C.byref(oi), C.byref(osd), threads)
-        dt = time.perf_counter() - t0
-    if v % 999983 != HASH_LARGE:
-        raise SystemExit(f"CPU baseline produced hash {v % 999983}, expected {HASH_LARGE}")
+        v = run()
+        times.append(time.perf_counter() - t0)
+        if v % 999983 != HASH_LARGE:
+            raise SystemExit(f"CPU baseline produced hash {v % 999983}, expected {HASH_LARGE}")
+    dt = sorted(times)[len(times) // 2]
     return {"value": round(LOOKUPS / dt, 1), "unit": "lookups/s", "cores": threads, "kind": kind,
-            "sample": f"all {LOOKUPS:,} lookups of the workload, one run of "
-                      f"run_event_based_simulation (openmp-threading, schedule(dynamic,100)), "
-                      f"{dt:.2f} s"}
+            "sample": f"all {LOOKUPS:,} lookups of the workload, median of {runs} runs of "
+                      f"run_event_based_simulation (openmp-threading, schedule(dynamic,100)) "
+                      f"on {threads} threads of {_cpu_model()}: "
+                      + ", ".join(f"{t:.2f}" for t in times) + " s"}
 
 
 def batched_variants(X, dev):
