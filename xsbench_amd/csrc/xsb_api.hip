@@ -112,7 +112,7 @@ static void release(Device *d)
 	batch_state_free(d);
 	if (d->history_graph.exec)
 		(void)hipGraphExecDestroy(d->history_graph.exec);
-	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_midx, d->d_grid,  d->d_ivl, d->d_ivl32, d->d_xs32, d->d_xs16,
+	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_midx, d->d_grid,  d->d_ivl, d->d_ivl32, d->d_ueg_acc, d->d_xs32, d->d_xs16,
 	                d->d_xs16_scale, d->d_num_nucs, d->d_mats, d->d_concs, d->d_concs32, d->d_acc,
 	                d->d_fftab};
 	for (void *p : ptrs)
@@ -484,6 +484,41 @@ __global__ void k_interval_records32(const double *__restrict__ grid, int64_t to
 	}
 }
 
+// Tables::ueg_acc: acc[b] = #{u : ueg[u] <= b / 2^16} (upper_bound by
+// bisection), b = 0..2^16.
+__global__ void k_ueg_acc(const double *__restrict__ ueg, int32_t n, int32_t *__restrict__ acc)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b > (1 << kUegAccBits))
+		return;
+	const double q = (double)b / (double)(1 << kUegAccBits);
+	int32_t lo = 0, hi = n;  // first index with ueg > q lies in [lo, hi]
+	while (lo < hi) {
+		const int32_t mid = lo + ((hi - lo) >> 1);
+		if (ueg[mid] > q)
+			hi = mid;
+		else
+			lo = mid + 1;
+	}
+	acc[b] = lo;
+}
+
+static int build_ueg_acc(Device *d)
+{
+	const int nb = (1 << kUegAccBits) + 1;
+	if (hipMalloc((void **)&d->d_ueg_acc, (size_t)nb * sizeof(int32_t)) != hipSuccess) {
+		(void)hipGetLastError();
+		d->d_ueg_acc = nullptr;
+		return fail(XSB_ERR_OOM, "hipMalloc of the unionized search brackets failed");
+	}
+	hipLaunchKernelGGL(k_ueg_acc, dim3((nb + 255) / 256), dim3(256), 0, d->stream, d->d_ueg,
+	                   (int32_t)d->T.n_union, d->d_ueg_acc);
+	XSB_HIP(hipGetLastError());
+	XSB_HIP(hipStreamSynchronize(d->stream));
+	d->T.ueg_acc = d->d_ueg_acc;
+	return XSB_OK;
+}
+
 static int build_interval_records(Device *d)
 {
 	Tables &T = d->T;
@@ -689,7 +724,7 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 
 	if ((rc = build_compact(d, t, flags, xs_max)))
 		return rc;
-	if (t->grid_type == XSB_UNIONIZED && (rc = build_interval_records(d)))
+	if (t->grid_type == XSB_UNIONIZED && ((rc = build_interval_records(d)) || (rc = build_ueg_acc(d))))
 		return rc;
 	T.xs32 = reinterpret_cast<const float *>(d->d_xs32);
 	T.xs16 = d->d_xs16;

@@ -27,6 +27,10 @@ struct Tables {
 	const double *__restrict__ ueg;         // unionized energies [n_union]
 	const int32_t *__restrict__ index_grid; // [n_union or hash_bins][n_iso]
 	const double *__restrict__ grid;        // FP64 AoS, 6 doubles per point
+	// unionized grid: ueg_acc[b] = #{u : ueg[u] <= b / 2^16}, b = 0..2^16 --
+	// the bisection of an energy in [b, b+1) / 2^16 starts from
+	// [ueg_acc[b], ueg_acc[b+1]] instead of the whole array
+	const int32_t *__restrict__ ueg_acc;
 	// unionized grid: FP64 interval records for the wave path, record
 	// nuc * n_gp + k = point k+1 (E, 5 XS), then point k+1 - point k
 	const double *__restrict__ ivl;
@@ -150,6 +154,25 @@ __device__ __forceinline__ int32_t grid_search(int32_t n, double q, const double
 	return lo;
 }
 
+// grid_search with the bracket of upper_bound(q) narrowed to [L, H] (the
+// count of A <= q lies in it): same result, fewer probes.  Reads A[L..H-1].
+__device__ __forceinline__ int32_t grid_search_in(int32_t n, double q,
+                                                  const double *__restrict__ A, int32_t L,
+                                                  int32_t H)
+{
+	int32_t lo = L - 1, hi = H;  // A[lo] <= q (or lo = -1), A[hi] > q (or hi = n)
+	while (hi - lo > 1) {
+		const int32_t mid = lo + ((hi - lo) >> 1);
+		if (A[mid] > q)
+			hi = mid;
+		else
+			lo = mid;
+	}
+	return min(max(hi - 1, 0), n - 2);
+}
+
+constexpr int kUegAccBits = 16;
+
 // bisection on energies stored with a stride (in doubles)
 template <int STRIDE>
 __device__ __forceinline__ int32_t grid_search_nuclide(double q, const double *__restrict__ E,
@@ -225,8 +248,13 @@ __device__ __forceinline__ int32_t nuclide_lower(const Tables &T, const double *
 template <int GRID>
 __device__ __forceinline__ int64_t lookup_index(const Tables &T, double p)
 {
-	if (GRID == kUnionized)
+	if (GRID == kUnionized) {
+		if (p >= 0.0 && p < 1.0) {
+			const int b = (int)(p * (double)(1 << kUegAccBits));  // exact scaling, truncation
+			return grid_search_in((int32_t)T.n_union, p, T.ueg, T.ueg_acc[b], T.ueg_acc[b + 1]);
+		}
 		return grid_search((int32_t)T.n_union, p, T.ueg);
+	}
 	if (GRID == kHash) {
 		const double q = p / T.hash_du;
 		if (!(q >= 0.0))

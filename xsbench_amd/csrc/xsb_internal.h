@@ -50,6 +50,7 @@ struct Device {
 	uint8_t *d_midx = nullptr;  // per-material unionized index tables (uint16 or int32)
 	double *d_grid = nullptr;
 	double *d_ivl = nullptr;    // FP64 interval records (Tables::ivl)
+	int32_t *d_ueg_acc = nullptr;  // unionized search brackets (Tables::ueg_acc)
 	double *d_ivl32 = nullptr;  // FP32 interval records (Tables::ivl32)
 	float *d_xs32 = nullptr;
 	uint16_t *d_xs16 = nullptr;
