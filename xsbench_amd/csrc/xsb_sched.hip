@@ -186,23 +186,37 @@ __global__ __launch_bounds__(256) void k_scan_parts(uint32_t *part, int nparts)
 	}
 }
 
+// The histogram length is a multiple of kBins, so a thread's 16 words are
+// either all in range or all out: read and written as four 16-byte vectors.
+static_assert(kScanItems == 16 && kBins % kScanItems == 0, "whole vectors per thread");
 __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *a, int64_t n,
                                                     const uint32_t *__restrict__ part)
 {
 	const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
-	uint32_t v[kScanItems];
+	const bool in = base < n;
+	uint4 w[kScanItems / 4];
+#pragma unroll
+	for (int q = 0; q < kScanItems / 4; q++)
+		w[q] = in ? reinterpret_cast<const uint4 *>(a + base)[q] : make_uint4(0u, 0u, 0u, 0u);
 	uint32_t s = 0;
-	for (int it = 0; it < kScanItems; it++) {
-		const int64_t i = base + it;
-		v[it] = i < n ? a[i] : 0u;
-		s += v[it];
-	}
+#pragma unroll
+	for (int q = 0; q < kScanItems / 4; q++)
+		s += w[q].x + w[q].y + w[q].z + w[q].w;
 	uint32_t off = block_excl_scan(s, nullptr) + part[blockIdx.x];
-	for (int it = 0; it < kScanItems; it++) {
-		const int64_t i = base + it;
-		if (i < n)
-			a[i] = off;
-		off += v[it];
+	if (!in)
+		return;
+#pragma unroll
+	for (int q = 0; q < kScanItems / 4; q++) {
+		uint4 o;
+		o.x = off;
+		off += w[q].x;
+		o.y = off;
+		off += w[q].y;
+		o.z = off;
+		off += w[q].z;
+		o.w = off;
+		off += w[q].w;
+		reinterpret_cast<uint4 *>(a + base)[q] = o;
 	}
 }
 
@@ -430,6 +444,8 @@ static int sort_pairs_t(hipStream_t s, int64_t n, int passes, bool hist0_ready, 
 {
 	const int ntiles = sort_tiles(n);
 	const int64_t hlen = sort_hist_len(n);
+	if (reinterpret_cast<uintptr_t>(hist) % 16 != 0)  // k_scan_apply's vectors
+		return fail(XSB_ERR_INVALID, "sort histogram not 16-byte aligned");
 	int rc;
 	for (int pass = 0; pass < passes; pass++) {
 		const int shift = kDigitBits * pass;
@@ -464,6 +480,8 @@ int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint32_t
 {
 	const int ntiles = sort_tiles(n);
 	const int64_t hlen = sort_hist_len(n);
+	if (reinterpret_cast<uintptr_t>(hist) % 16 != 0)  // k_scan_apply's vectors
+		return fail(XSB_ERR_INVALID, "sort histogram not 16-byte aligned");
 	int rc;
 	for (int pass = 0; pass < passes; pass++) {
 		const int shift = kDigitBits * pass;
