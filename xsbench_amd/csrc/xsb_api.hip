@@ -121,6 +121,8 @@ static void release(Device *d)
 	for (auto &s : d->scratch)
 		if (s.ptr)
 			(void)hipFree(s.ptr);
+	if (d->h_out)
+		(void)hipHostFree(d->h_out);
 	for (auto &e : d->ev)
 		if (e)
 			(void)hipEventDestroy(e);
@@ -672,6 +674,12 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 		return fail(XSB_ERR_OOM, "accumulator allocation failed");
 	}
 	XSB_HIP(hipMemset(d->d_acc, 0, 64 * sizeof(unsigned long long)));
+	if (hipHostMalloc((void **)&d->h_out, 2 * sizeof(unsigned long long) + 4 * 1024,
+	                  hipHostMallocDefault) != hipSuccess) {
+		(void)hipGetLastError();
+		d->h_out = nullptr;
+		return fail(XSB_ERR_OOM, "pinned host result buffer allocation failed");
+	}
 
 	// integrity checks that protect the kernels' memory accesses
 	unsigned long long bad = 0;
@@ -880,11 +888,13 @@ static int event_impl(Device *d, const xsb_event_args *a, xsb_result *r)
 			return rc;
 	}
 	XSB_HIP(hipEventRecord(d->ev[3], s));
-	unsigned long long acc[2];
-	XSB_HIP(hipMemcpyAsync(acc, d->d_acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+	XSB_HIP(hipMemcpyAsync(d->h_out, d->d_acc, 2 * sizeof(unsigned long long),
+	                       hipMemcpyDeviceToHost, s));
 	XSB_HIP(hipStreamSynchronize(s));
-	r->vsum = acc[0];
-	r->n_fallback = (int64_t)acc[1];
+	r->vsum = d->h_out[0];
+	r->n_fallback = (int64_t)d->h_out[1];
+	if (k != XSB_K_BASELINE && k != XSB_K_SPLIT)
+		finish_sorted_event(d, a, r);
 	r->ms_total = elapsed(d->ev[0], d->ev[3]);
 	r->ms_sample = elapsed(d->ev[0], d->ev[1]);
 	r->ms_lookup = elapsed(d->ev[1], d->ev[2]);

@@ -60,6 +60,9 @@ struct Device {
 	double *d_concs = nullptr;
 	float *d_concs32 = nullptr;
 	unsigned long long *d_acc = nullptr;  // 64 accumulators
+	// pinned host words the event path copies its results into (2
+	// accumulators, then kBins digit starts), so one synchronisation ends a call
+	unsigned long long *h_out = nullptr;
 	uint64_t *d_fftab = nullptr;          // LCG skip-ahead table
 	int64_t bytes_fp64 = 0, bytes_fp32 = 0, bytes_bf16 = 0, bytes_index = 0, bytes_union = 0;
 	int64_t bytes_midx = 0, bytes_ivl = 0;
@@ -103,8 +106,12 @@ int launch_balg(Device *d, int64_t first, int64_t n, uint64_t seed, int precisio
 int launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);
 
 // ---- sorted schedules (xsb_sched.hip) ----
+// Enqueues the sorted schedule (no synchronisation; the digit starts are
+// copied to h_out + 2), then, once the stream has been synchronised,
+// finish_sorted_event fills r->material_counts from them.
 int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, xsb_result *r,
                      unsigned long long *vsum, unsigned long long *nfb);
+void finish_sorted_event(const Device *d, const xsb_event_args *a, xsb_result *r);
 
 // ---- grid construction on the device (xsb_gridinit.hip): allocates and
 // fills d_grid, d_ueg (unionized) and d_index (rows padded to `pitch`), and

@@ -554,11 +554,24 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	XSB_HIP(hipGetLastError());
 	XSB_HIP(hipEventRecord(d->ev[2], s));
 
-	// material counts from the last pass's digit starts
+	// the last pass's digit starts, for the material counts
+	static_assert(kBins * sizeof(uint32_t) <= 4 * 1024, "fits the pinned result buffer");
+	XSB_HIP(hipMemcpyAsync(d->h_out + 2, starts, kBins * sizeof(uint32_t), hipMemcpyDeviceToHost,
+	                       s));
+	return XSB_OK;
+}
+
+void finish_sorted_event(const Device *d, const xsb_event_args *a, xsb_result *r)
+{
 	static_assert(kEnergyBits + 4 <= 2 * kDigitBits, "material-energy key fits two digits");
-	uint32_t st[kBins];
-	XSB_HIP(hipMemcpyAsync(st, starts, sizeof(st), hipMemcpyDeviceToHost, s));
-	XSB_HIP(hipStreamSynchronize(s));
+	const int64_t n = a->lookups;
+	if (n == 0)
+		return;
+	const int k = r->kernel_id;
+	const int mode = (k == XSB_K_MAT_E_SORT) ? kModeMatE
+	                 : (k == XSB_K_FUEL_LAUNCH || k == XSB_K_FUEL_SORT) ? kModeFuel
+	                                                                    : kModeMat;
+	const uint32_t *st = reinterpret_cast<const uint32_t *>(d->h_out + 2);
 	if (mode != kModeFuel) {
 		for (int dgt = 0; dgt < kBins; dgt++) {
 			const uint32_t next = (dgt == kBins - 1) ? (uint32_t)n : st[dgt + 1];
@@ -569,7 +582,6 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	} else {
 		r->material_counts[0] = (int64_t)st[1];
 	}
-	return XSB_OK;
 }
 
 }  // namespace xsb
