@@ -104,6 +104,23 @@ def test_large_unionized_event_hash_every_schedule(k):
         assert sum(r["material_counts"]) == 17_000_000
 
 
+@pytest.mark.parametrize("k", [X.K_MAT_LAUNCH, X.K_FUEL_LAUNCH, X.K_MAT_SORT, X.K_FUEL_SORT,
+                               X.K_MAT_E_SORT])
+def test_sorted_schedule_material_counts(orc, k):
+    """the per-material counts a sorted schedule reports (from the sort's
+    digit starts) against the oracle's sampled materials"""
+    inp, sd, dev = data("small_unionized")
+    oi, osd = as_oracle(inp, sd)
+    first, n = 777, 1 << 18
+    _, m, _, _, _ = orc.event_detail(oi, osd, np.arange(first, first + n))
+    want = np.bincount(m, minlength=12)
+    got = np.array(dev.event(n, first, kernel_id=k)["material_counts"])
+    if k in (X.K_FUEL_LAUNCH, X.K_FUEL_SORT):
+        assert got[0] == want[0] and got[1:].sum() == 0
+    else:
+        assert np.array_equal(got, want)
+
+
 @pytest.mark.parametrize("name,k", [("large_hash", X.K_BASELINE), ("large_hash", X.K_MAT_E_SORT),
                                     ("small_unionized", X.K_BASELINE), ("small_hash", X.K_MAT_E_SORT),
                                     ("small_nuclide", X.K_BASELINE), ("small_nuclide", X.K_MAT_E_SORT)])
