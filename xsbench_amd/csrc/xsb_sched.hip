@@ -36,9 +36,11 @@ __device__ __forceinline__ uint32_t sample_key(const Tables &T, uint64_t s, int 
 
 // Keys + ids for lookups [first, first+n), and the first pass's per-tile
 // digit histogram hist[digit * ntiles + tile].
+// Output: {key, lookup id} pairs, interleaved (one 8-byte word per lookup:
+// the sort moves each element with one load and one store).
 __global__ __launch_bounds__(kSortBlock) void k_keys(Tables T, int64_t first, int64_t n,
-                                                     uint64_t seed0, int mode, uint32_t *keys,
-                                                     uint32_t *vals, uint32_t *hist, int ntiles)
+                                                     uint64_t seed0, int mode, uint2 *pairs,
+                                                     uint32_t *hist, int ntiles)
 {
 	__shared__ uint32_t h[kBins];
 	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
@@ -60,8 +62,7 @@ __global__ __launch_bounds__(kSortBlock) void k_keys(Tables T, int64_t first, in
 		if (e < n) {
 			const uint64_t s = (lane_skip.x * s0 + lane_skip.y) & kLcgMask;
 			const uint32_t k = sample_key(T, s, mode);
-			keys[e] = k;
-			vals[e] = (uint32_t)e;
+			pairs[e] = make_uint2(k, (uint32_t)e);
 			atomicAdd(&h[k & kDigitMask], 1u);
 		}
 		s0 = (item_skip.x * s0 + item_skip.y) & kLcgMask;
@@ -85,6 +86,31 @@ __global__ __launch_bounds__(kSortBlock) void k_hist(const K *__restrict__ keys,
 	for (int it = 0; it < kSortItems; it++) {
 		const int64_t e = base + tile_pos(wave, it, lane);
 		dg[it] = e < n ? (uint32_t)(keys[e] >> shift) & kDigitMask : (uint32_t)kBins;
+	}
+#pragma unroll
+	for (int it = 0; it < kSortItems; it++)
+		if (dg[it] < (uint32_t)kBins)
+			atomicAdd(&h[dg[it]], 1u);
+	__syncthreads();
+	write_tile_hist(h, hist, ntiles);
+}
+
+// k_hist over interleaved {key, id} pairs.
+__global__ __launch_bounds__(kSortBlock) void k_hist_pairs(const uint2 *__restrict__ pairs,
+                                                           int64_t n, int shift, uint32_t *hist,
+                                                           int ntiles)
+{
+	__shared__ uint32_t h[kBins];
+	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
+		h[d] = 0;
+	__syncthreads();
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int64_t base = (int64_t)blockIdx.x * kTile;
+	uint32_t dg[kSortItems];
+#pragma unroll
+	for (int it = 0; it < kSortItems; it++) {
+		const int64_t e = base + tile_pos(wave, it, lane);
+		dg[it] = e < n ? (pairs[e].x >> shift) & kDigitMask : (uint32_t)kBins;
 	}
 #pragma unroll
 	for (int it = 0; it < kSortItems; it++)
@@ -293,6 +319,8 @@ constexpr int kRunsBlock = 1024;                 // 16 waves: two blocks fill a 
 constexpr int kRunsItems = kTile / kRunsBlock;   // 8 elements per thread
 static_assert(kBins == kRunsBlock, "one digit per thread in the local scan");
 
+// PAIRED: kin / kout hold interleaved uint2 {key, id} (vin / vout unused).
+template <bool PAIRED>
 __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint32_t *__restrict__ kin,
                                                              const uint32_t *__restrict__ vin,
                                                              uint32_t *__restrict__ kout,
@@ -313,8 +341,14 @@ __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint32_t *__r
 	for (int it = 0; it < kRunsItems; it++) {
 		const int i = it * kRunsBlock + threadIdx.x;
 		if (i < m) {
-			kk[it] = kin[base + i];
-			vv[it] = vin[base + i];
+			if (PAIRED) {
+				const uint2 q = reinterpret_cast<const uint2 *>(kin)[base + i];
+				kk[it] = q.x;
+				vv[it] = q.y;
+			} else {
+				kk[it] = kin[base + i];
+				vv[it] = vin[base + i];
+			}
 		}
 	}
 #pragma unroll
@@ -346,8 +380,12 @@ __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint32_t *__r
 		if (i < m) {
 			const uint32_t key = sk[i];
 			const uint32_t g = gbase[(key >> shift) & kDigitMask] + (uint32_t)i;
-			kout[g] = key;
-			vout[g] = sv[i];
+			if (PAIRED) {
+				reinterpret_cast<uint2 *>(kout)[g] = make_uint2(key, sv[i]);
+			} else {
+				kout[g] = key;
+				vout[g] = sv[i];
+			}
 		}
 	}
 }
@@ -359,7 +397,8 @@ __global__ void k_digit_starts(const uint32_t *__restrict__ hist, int ntiles, ui
 		out[d] = hist[(int64_t)d * ntiles];
 }
 
-// Lookups in sorted order: position i holds lookup id ids[i]; (E, mat) are
+// Lookups in sorted order: position i holds lookup id ids[2 i] (the ids of
+// the sorted {key, id} pairs); (E, mat) are
 // re-derived from the id's LCG state.  Lanes past n repeat lookup n-1 (in the
 // same wave) and contribute nothing, so every lane stays active for the
 // wave-wide search and the wave-uniform material path.
@@ -374,7 +413,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 {
 	const int64_t i = (int64_t)xcd_block(kXcdRun) * 256 + threadIdx.x;
 	const bool valid = i < n;
-	const uint32_t id = ids[valid ? i : n - 1];
+	const uint32_t id = ids[(valid ? i : n - 1) * 2];  // ids interleaved with keys
 	uint64_t s = ff_jump(seed0, 2ULL * (uint64_t)(first + id), T.ff_tab);
 	const double p = lcg_double(s);
 	const int mat = pick_mat(s, T.thr);
@@ -492,8 +531,8 @@ int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint32_t
 		}
 		if ((rc = scan_hist(hist, hlen, part, s)))
 			return rc;
-		hipLaunchKernelGGL(k_scatter_runs, dim3(ntiles), dim3(kRunsBlock), 0, s, *k, *v, *k2, *v2,
-		                   n, shift, hist, ntiles);
+		hipLaunchKernelGGL(k_scatter_runs<false>, dim3(ntiles), dim3(kRunsBlock), 0, s, *k, *v, *k2,
+		                   *v2, n, shift, hist, ntiles);
 		XSB_HIP(hipGetLastError());
 		uint32_t *t = *k;
 		*k = *k2;
@@ -501,6 +540,35 @@ int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint32_t
 		t = *v;
 		*v = *v2;
 		*v2 = t;
+	}
+	return XSB_OK;
+}
+
+// order_pairs on interleaved {key, id} pairs.
+static int order_pairs_interleaved(hipStream_t s, int64_t n, int passes, bool hist0_ready,
+                                   uint2 **p, uint2 **p2, uint32_t *hist, uint32_t *part)
+{
+	const int ntiles = sort_tiles(n);
+	const int64_t hlen = sort_hist_len(n);
+	if (reinterpret_cast<uintptr_t>(hist) % 16 != 0)  // k_scan_apply's vectors
+		return fail(XSB_ERR_INVALID, "sort histogram not 16-byte aligned");
+	int rc;
+	for (int pass = 0; pass < passes; pass++) {
+		const int shift = kDigitBits * pass;
+		if (pass > 0 || !hist0_ready) {
+			hipLaunchKernelGGL(k_hist_pairs, dim3(ntiles), dim3(kSortBlock), 0, s, *p, n, shift,
+			                   hist, ntiles);
+			XSB_HIP(hipGetLastError());
+		}
+		if ((rc = scan_hist(hist, hlen, part, s)))
+			return rc;
+		hipLaunchKernelGGL(k_scatter_runs<true>, dim3(ntiles), dim3(kRunsBlock), 0, s,
+		                   (const uint32_t *)*p, nullptr, (uint32_t *)*p2, nullptr, n, shift, hist,
+		                   ntiles);
+		XSB_HIP(hipGetLastError());
+		uint2 *t = *p;
+		*p = *p2;
+		*p2 = t;
 	}
 	return XSB_OK;
 }
@@ -534,14 +602,15 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	int rc = scratch_reserve(d, 0, bytes, &buf);
 	if (rc)
 		return rc;
-	uint32_t *kA = (uint32_t *)buf, *vA = kA + n, *kB = vA + n, *vB = kB + n;
-	uint32_t *hist = vB + n, *part = hist + hlen, *starts = part + nparts + 16;
+	uint2 *pA = (uint2 *)buf, *pB = pA + n;
+	uint32_t *hist = (uint32_t *)(pB + n), *part = hist + hlen, *starts = part + nparts + 16;
 
 	hipLaunchKernelGGL(k_keys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, a->first_lookup, n,
-	                   a->seed, mode, kA, vA, hist, ntiles);
+	                   a->seed, mode, pA, hist, ntiles);
 	XSB_HIP(hipGetLastError());
-	if ((rc = order_pairs(s, n, passes, true, &kA, &vA, &kB, &vB, hist, part)))
+	if ((rc = order_pairs_interleaved(s, n, passes, true, &pA, &pB, hist, part)))
 		return rc;
+	const uint32_t *vA = reinterpret_cast<const uint32_t *>(pA) + 1;
 	hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(256), 0, s, hist, ntiles, starts);
 	XSB_HIP(hipGetLastError());
 	XSB_HIP(hipEventRecord(d->ev[1], s));
