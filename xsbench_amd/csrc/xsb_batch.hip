@@ -266,8 +266,8 @@ __device__ __forceinline__ int batch_of(const uint32_t *off, int nb, uint32_t g)
 }
 
 struct SortedQueue {
-	const uint32_t *keys;      // sorted keys: material in bits [ebits, ebits + 4)
-	const uint32_t *ids;       // sorted global lookup ids
+	const uint2 *pairs;        // sorted {key, global lookup id}: material in key
+	                           // bits [ebits, ebits + 4)
 	int64_t n;
 	int64_t n_items;
 	int32_t ebits;
@@ -295,8 +295,7 @@ __global__ void k_breset(SortedQueue q, int nb)
 }
 
 __global__ __launch_bounds__(kSortBlock) void k_bkeys(Tables T, BatchView B, int64_t n, int ebits,
-                                                      uint32_t *keys, uint32_t *vals,
-                                                      uint32_t *hist, int ntiles)
+                                                      uint2 *pairs, uint32_t *hist, int ntiles)
 {
 	__shared__ uint32_t h[kBins];
 	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
@@ -320,8 +319,7 @@ __global__ __launch_bounds__(kSortBlock) void k_bkeys(Tables T, BatchView B, int
 			uint32_t eb = (uint32_t)(p * (double)(1u << ebits));  // exact scaling, truncation
 			eb = eb < emax ? eb : emax;
 			const uint32_t key = (B.win[b] << (4 + ebits)) | ((uint32_t)mat << ebits) | eb;
-			keys[e] = key;
-			vals[e] = g;
+			pairs[e] = make_uint2(key, g);
 			atomicAdd(&h[key & kDigitMask], 1u);
 		}
 	}
@@ -395,7 +393,8 @@ __global__ __launch_bounds__(kBatchBlock) void k_bsorted(Tables T, BatchView B, 
 			const int64_t pos = r + threadIdx.x;
 			const bool valid = pos < p1;
 			const int64_t pp = valid ? pos : p1 - 1;
-			const uint32_t key = q.keys[pp], g = q.ids[pp];
+			const uint2 kv = q.pairs[pp];
+			const uint32_t key = kv.x, g = kv.y;
 			const int mat = (int)((key >> q.ebits) & 15u);
 			const int b = batch_of(off, nb, g);
 			uint64_t s = ff_jump(B.seeds[b], 2ULL * (uint64_t)(g - off[b]), T.ff_tab);
@@ -590,21 +589,20 @@ struct TableLayout {
 static int enqueue_pipeline(Device *d, const PipeShape &ps, const BatchView &B, const SortedQueue &q0,
                             uint64_t *clk, hipStream_t s)
 {
-	uint32_t *kA = (uint32_t *)ps.sortbuf, *vA = kA + ps.n, *kB = vA + ps.n, *vB = kB + ps.n;
-	uint32_t *hist = vB + ps.n, *part = hist + sort_hist_len(ps.n);
+	uint2 *pA = (uint2 *)ps.sortbuf, *pB = pA + ps.n;
+	uint32_t *hist = (uint32_t *)(pB + ps.n), *part = hist + sort_hist_len(ps.n);
 	const int ntiles = sort_tiles(ps.n);
 	hipLaunchKernelGGL(k_breset, dim3((ps.nb + 255) / 256), dim3(256), 0, s, q0, ps.nb);
 	hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, clk);
-	hipLaunchKernelGGL(k_bkeys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, B, ps.n, ps.ebits, kA, vA,
+	hipLaunchKernelGGL(k_bkeys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, B, ps.n, ps.ebits, pA,
 	                   hist, ntiles);
 	XSB_HIP(hipGetLastError());
-	int rc = order_pairs(s, ps.n, ps.passes, true, &kA, &vA, &kB, &vB, hist, part);
+	int rc = order_pairs(s, ps.n, ps.passes, true, &pA, &pB, hist, part);
 	if (rc)
 		return rc;
 	hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, clk + 1);
 	SortedQueue q = q0;
-	q.keys = kA;
-	q.ids = vA;
+	q.pairs = pA;
 	q.n = ps.n;
 	q.tile = ps.plan.tile;
 	q.n_items = (ps.n + ps.plan.tile - 1) / ps.plan.tile;

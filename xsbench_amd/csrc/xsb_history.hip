@@ -34,8 +34,7 @@ __global__ __launch_bounds__(kHBlock) void k_hinit(Tables T, int64_t P, int look
 // key = material << 16 | top 16 bits of the next energy; value = particle
 __global__ __launch_bounds__(kSortBlock) void k_hkeys(Tables T, int64_t P,
                                                       const uint64_t *__restrict__ seed,
-                                                      uint32_t *keys, uint32_t *vals,
-                                                      uint32_t *hist, int ntiles)
+                                                      uint2 *pairs, uint32_t *hist, int ntiles)
 {
 	__shared__ uint32_t h[kBins];
 	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
@@ -52,8 +51,7 @@ __global__ __launch_bounds__(kSortBlock) void k_hkeys(Tables T, int64_t P,
 			uint32_t eb = (uint32_t)(p * (double)(1u << kHEnergyBits));
 			eb = eb < (1u << kHEnergyBits) ? eb : (1u << kHEnergyBits) - 1u;
 			const uint32_t key = ((uint32_t)mat << kHEnergyBits) | eb;
-			keys[e] = key;
-			vals[e] = (uint32_t)e;
+			pairs[e] = make_uint2(key, (uint32_t)e);
 			atomicAdd(&h[key & kDigitMask], 1u);
 		}
 	}
@@ -64,10 +62,10 @@ __global__ __launch_bounds__(kSortBlock) void k_hkeys(Tables T, int64_t P,
 // One round in sorted order (Simulation.c:176-234 for one i): draw the
 // energy and material, look up, add argmax + 1, fast-forward the state by
 // the number of channels above 1.0.  Lanes past P repeat the last particle
-// and write nothing.
+// and write nothing.  ids: the sorted {key, particle} pairs.
 template <int GRID, int PREC, int CHUNK>
 __global__ __launch_bounds__(kHBlock) void k_hround(Tables T, int64_t P,
-                                                    const uint32_t *__restrict__ ids,
+                                                    const uint2 *__restrict__ ids,
                                                     uint64_t *__restrict__ seed,
                                                     uint32_t *__restrict__ psum,
                                                     unsigned long long *vsum,
@@ -75,7 +73,7 @@ __global__ __launch_bounds__(kHBlock) void k_hround(Tables T, int64_t P,
 {
 	const int64_t i = (int64_t)xcd_block(kHXcdRun) * kHBlock + threadIdx.x;
 	const bool valid = i < P;
-	const uint32_t q = ids[valid ? i : P - 1];
+	const uint32_t q = ids[valid ? i : P - 1].y;
 	uint64_t s = seed[q];
 	const double p = lcg_double(s);
 	const int mat = pick_mat(s, T.thr);
@@ -125,7 +123,7 @@ __global__ void k_hreset(unsigned long long *acc)
 }
 
 template <int GRID, int PREC>
-static void launch_round(Device *d, hipStream_t s, int64_t P, const uint32_t *ids, uint64_t *seed,
+static void launch_round(Device *d, hipStream_t s, int64_t P, const uint2 *ids, uint64_t *seed,
                          uint32_t *psum, unsigned long long *acc)
 {
 	const unsigned nb = (unsigned)((P + kHBlock - 1) / kHBlock);
@@ -135,7 +133,7 @@ static void launch_round(Device *d, hipStream_t s, int64_t P, const uint32_t *id
 }
 
 template <int PREC>
-static void launch_round_grid(Device *d, hipStream_t s, int64_t P, const uint32_t *ids,
+static void launch_round_grid(Device *d, hipStream_t s, int64_t P, const uint2 *ids,
                               uint64_t *seed, uint32_t *psum, unsigned long long *acc)
 {
 	switch (d->T.grid_type) {
@@ -147,7 +145,8 @@ static void launch_round_grid(Device *d, hipStream_t s, int64_t P, const uint32_
 
 struct HistoryBuffers {
 	uint64_t *seed;
-	uint32_t *psum, *kA, *vA, *kB, *vB, *hist, *part;
+	uint32_t *psum, *hist, *part;
+	uint2 *pA, *pB;  // {key, particle} pairs, two buffers
 	uint64_t *out_sum, *out_seed;
 	unsigned long long *acc;
 };
@@ -163,19 +162,19 @@ static int enqueue_rounds(Device *d, hipStream_t s, int64_t P, int lookups, uint
 	                   B.psum);
 	XSB_HIP(hipGetLastError());
 	for (int r = 0; r < lookups; r++) {
-		uint32_t *kA = B.kA, *vA = B.vA, *kB = B.kB, *vB = B.vB;
-		hipLaunchKernelGGL(k_hkeys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, P, B.seed, kA, vA,
+		uint2 *pA = B.pA, *pB = B.pB;
+		hipLaunchKernelGGL(k_hkeys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, P, B.seed, pA,
 		                   B.hist, ntiles);
 		XSB_HIP(hipGetLastError());
-		int rc = order_pairs(s, P, 2, true, &kA, &vA, &kB, &vB, B.hist, B.part);
+		int rc = order_pairs(s, P, 2, true, &pA, &pB, B.hist, B.part);
 		if (rc)
 			return rc;
 		if (prec == XSB_XS_FP64)
-			launch_round_grid<XSB_XS_FP64>(d, s, P, vA, B.seed, B.psum, B.acc);
+			launch_round_grid<XSB_XS_FP64>(d, s, P, pA, B.seed, B.psum, B.acc);
 		else if (prec == XSB_XS_FP32)
-			launch_round_grid<XSB_XS_FP32>(d, s, P, vA, B.seed, B.psum, B.acc);
+			launch_round_grid<XSB_XS_FP32>(d, s, P, pA, B.seed, B.psum, B.acc);
 		else
-			launch_round_grid<XSB_XS_BF16>(d, s, P, vA, B.seed, B.psum, B.acc);
+			launch_round_grid<XSB_XS_BF16>(d, s, P, pA, B.seed, B.psum, B.acc);
 		XSB_HIP(hipGetLastError());
 	}
 	if (B.out_sum || B.out_seed) {
@@ -207,10 +206,8 @@ int run_history_rounds(Device *d, const xsb_history_args *a, xsb_result *r)
 	};
 	B.seed = (uint64_t *)take((size_t)P * 8);
 	B.psum = (uint32_t *)take((size_t)P * 4);
-	B.kA = (uint32_t *)take((size_t)P * 4);
-	B.vA = (uint32_t *)take((size_t)P * 4);
-	B.kB = (uint32_t *)take((size_t)P * 4);
-	B.vB = (uint32_t *)take((size_t)P * 4);
+	B.pA = (uint2 *)take((size_t)P * 8);
+	B.pB = (uint2 *)take((size_t)P * 8);
 	B.hist = (uint32_t *)take((size_t)sort_hist_len(P) * 4);
 	B.part = (uint32_t *)take((size_t)sort_parts(P) * 4);
 	B.out_sum = a->out_particle_sum ? (uint64_t *)take((size_t)P * 8) : nullptr;

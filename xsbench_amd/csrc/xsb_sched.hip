@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ ki
 	}
 }
 
-// Scatter of one digit for the lookup schedules, which need the order only
+// Scatter of one digit of {key, value} pairs for the lookup schedules, which need the order only
 // for locality: a tile's elements are grouped by digit in LDS (ranks from
 // LDS counters, so equal digits of one tile may swap places), then written
 // out in runs, consecutive lanes storing consecutive addresses.  After the
@@ -319,13 +319,9 @@ constexpr int kRunsBlock = 1024;                 // 16 waves: two blocks fill a 
 constexpr int kRunsItems = kTile / kRunsBlock;   // 8 elements per thread
 static_assert(kBins == kRunsBlock, "one digit per thread in the local scan");
 
-// PAIRED: kin / kout hold interleaved uint2 {key, id} (vin / vout unused).
-template <bool PAIRED>
-__global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint32_t *__restrict__ kin,
-                                                             const uint32_t *__restrict__ vin,
-                                                             uint32_t *__restrict__ kout,
-                                                             uint32_t *__restrict__ vout,
-                                                             int64_t n, int shift,
+__global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__restrict__ pin,
+                                                             uint2 *__restrict__ pout, int64_t n,
+                                                             int shift,
                                                              const uint32_t *__restrict__ hist,
                                                              int ntiles)
 {
@@ -341,14 +337,9 @@ __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint32_t *__r
 	for (int it = 0; it < kRunsItems; it++) {
 		const int i = it * kRunsBlock + threadIdx.x;
 		if (i < m) {
-			if (PAIRED) {
-				const uint2 q = reinterpret_cast<const uint2 *>(kin)[base + i];
-				kk[it] = q.x;
-				vv[it] = q.y;
-			} else {
-				kk[it] = kin[base + i];
-				vv[it] = vin[base + i];
-			}
+			const uint2 q = pin[base + i];
+			kk[it] = q.x;
+			vv[it] = q.y;
 		}
 	}
 #pragma unroll
@@ -380,12 +371,7 @@ __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint32_t *__r
 		if (i < m) {
 			const uint32_t key = sk[i];
 			const uint32_t g = gbase[(key >> shift) & kDigitMask] + (uint32_t)i;
-			if (PAIRED) {
-				reinterpret_cast<uint2 *>(kout)[g] = make_uint2(key, sv[i]);
-			} else {
-				kout[g] = key;
-				vout[g] = sv[i];
-			}
+			pout[g] = make_uint2(key, sv[i]);
 		}
 	}
 }
@@ -514,39 +500,8 @@ int sort_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint32_t 
 	return sort_pairs_t<uint32_t>(s, n, passes, hist0_ready, k, v, k2, v2, hist, part);
 }
 
-int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint32_t **k, uint32_t **v,
-                uint32_t **k2, uint32_t **v2, uint32_t *hist, uint32_t *part)
-{
-	const int ntiles = sort_tiles(n);
-	const int64_t hlen = sort_hist_len(n);
-	if (reinterpret_cast<uintptr_t>(hist) % 16 != 0)  // k_scan_apply's vectors
-		return fail(XSB_ERR_INVALID, "sort histogram not 16-byte aligned");
-	int rc;
-	for (int pass = 0; pass < passes; pass++) {
-		const int shift = kDigitBits * pass;
-		if (pass > 0 || !hist0_ready) {
-			hipLaunchKernelGGL(k_hist<uint32_t>, dim3(ntiles), dim3(kSortBlock), 0, s, *k, n, shift,
-			                   hist, ntiles);
-			XSB_HIP(hipGetLastError());
-		}
-		if ((rc = scan_hist(hist, hlen, part, s)))
-			return rc;
-		hipLaunchKernelGGL(k_scatter_runs<false>, dim3(ntiles), dim3(kRunsBlock), 0, s, *k, *v, *k2,
-		                   *v2, n, shift, hist, ntiles);
-		XSB_HIP(hipGetLastError());
-		uint32_t *t = *k;
-		*k = *k2;
-		*k2 = t;
-		t = *v;
-		*v = *v2;
-		*v2 = t;
-	}
-	return XSB_OK;
-}
-
-// order_pairs on interleaved {key, id} pairs.
-static int order_pairs_interleaved(hipStream_t s, int64_t n, int passes, bool hist0_ready,
-                                   uint2 **p, uint2 **p2, uint32_t *hist, uint32_t *part)
+int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint2 **p,
+                            uint2 **p2, uint32_t *hist, uint32_t *part)
 {
 	const int ntiles = sort_tiles(n);
 	const int64_t hlen = sort_hist_len(n);
@@ -562,9 +517,8 @@ static int order_pairs_interleaved(hipStream_t s, int64_t n, int passes, bool hi
 		}
 		if ((rc = scan_hist(hist, hlen, part, s)))
 			return rc;
-		hipLaunchKernelGGL(k_scatter_runs<true>, dim3(ntiles), dim3(kRunsBlock), 0, s,
-		                   (const uint32_t *)*p, nullptr, (uint32_t *)*p2, nullptr, n, shift, hist,
-		                   ntiles);
+		hipLaunchKernelGGL(k_scatter_runs, dim3(ntiles), dim3(kRunsBlock), 0, s, *p, *p2, n, shift,
+		                   hist, ntiles);
 		XSB_HIP(hipGetLastError());
 		uint2 *t = *p;
 		*p = *p2;
@@ -608,7 +562,7 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	hipLaunchKernelGGL(k_keys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, a->first_lookup, n,
 	                   a->seed, mode, pA, hist, ntiles);
 	XSB_HIP(hipGetLastError());
-	if ((rc = order_pairs_interleaved(s, n, passes, true, &pA, &pB, hist, part)))
+	if ((rc = order_pairs(s, n, passes, true, &pA, &pB, hist, part)))
 		return rc;
 	const uint32_t *vA = reinterpret_cast<const uint32_t *>(pA) + 1;
 	hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(256), 0, s, hist, ntiles, starts);
