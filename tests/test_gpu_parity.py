@@ -104,6 +104,20 @@ def test_large_unionized_event_hash_every_schedule(k):
         assert sum(r["material_counts"]) == 17_000_000
 
 
+@pytest.mark.parametrize("n", [1, 3, 8191, 8193, 100_003])
+def test_sorted_schedule_odd_and_partial_tiles(orc, n):
+    """odd lookup counts and partial sort tiles (pair buffers, histogram tails)
+    on the material-energy schedule, per lookup against the oracle"""
+    inp, sd, dev = data("small_unionized")
+    oi, osd = as_oracle(inp, sd)
+    first = 424_242
+    E, m, idx, macro, am = orc.event_detail(oi, osd, np.arange(first, first + n))
+    r, out = dev.event(n, first, kernel_id=X.K_MAT_E_SORT, detail=True)
+    assert same_bits(out["macro"], macro) and np.array_equal(out["argmax"], am)
+    assert r["vsum"] == int(am.sum()) + n
+    assert sum(r["material_counts"]) == n
+
+
 @pytest.mark.parametrize("k", [X.K_MAT_LAUNCH, X.K_FUEL_LAUNCH, X.K_MAT_SORT, X.K_FUEL_SORT,
                                X.K_MAT_E_SORT])
 def test_sorted_schedule_material_counts(orc, k):

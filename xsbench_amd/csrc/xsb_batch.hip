@@ -589,8 +589,9 @@ struct TableLayout {
 static int enqueue_pipeline(Device *d, const PipeShape &ps, const BatchView &B, const SortedQueue &q0,
                             uint64_t *clk, hipStream_t s)
 {
-	uint2 *pA = (uint2 *)ps.sortbuf, *pB = pA + ps.n;
-	uint32_t *hist = (uint32_t *)(pB + ps.n), *part = hist + sort_hist_len(ps.n);
+	const int64_t n2 = (ps.n + 1) & ~(int64_t)1;  // pair buffers start 16-byte aligned
+	uint2 *pA = (uint2 *)ps.sortbuf, *pB = pA + n2;
+	uint32_t *hist = (uint32_t *)(pB + n2), *part = hist + sort_hist_len(ps.n);
 	const int ntiles = sort_tiles(ps.n);
 	hipLaunchKernelGGL(k_breset, dim3((ps.nb + 255) / 256), dim3(256), 0, s, q0, ps.nb);
 	hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, clk);
@@ -642,7 +643,7 @@ static int make_shape(Device *d, int nb, int64_t n, int windows, int prec, const
 	ps->grid = (ps->grid + kXcds - 1) / kXcds * kXcds;
 	ps->tables = tables;
 	const size_t sort_bytes =
-	    (size_t)n * 16 + (size_t)sort_hist_len(n) * 4 + (size_t)sort_parts(n) * 4 + 256;
+	    (size_t)(n + 1) * 16 + (size_t)sort_hist_len(n) * 4 + (size_t)sort_parts(n) * 4 + 256;
 	return scratch_reserve(d, 2, sort_bytes, &ps->sortbuf);
 }
 

@@ -104,13 +104,20 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_pairs(const uint2 *__restri
 	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
 		h[d] = 0;
 	__syncthreads();
-	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	// a histogram does not depend on the order: two pairs per 16-byte load
 	const int64_t base = (int64_t)blockIdx.x * kTile;
+	const uint4 *__restrict__ q = reinterpret_cast<const uint4 *>(pairs + base);
 	uint32_t dg[kSortItems];
 #pragma unroll
-	for (int it = 0; it < kSortItems; it++) {
-		const int64_t e = base + tile_pos(wave, it, lane);
-		dg[it] = e < n ? (pairs[e].x >> shift) & kDigitMask : (uint32_t)kBins;
+	for (int it = 0; it < kSortItems / 2; it++) {
+		const int64_t e = base + 2 * (it * kSortBlock + threadIdx.x);  // pairs e, e + 1
+		uint4 w = make_uint4(0u, 0u, 0u, 0u);
+		if (e + 1 < n)
+			w = q[it * kSortBlock + threadIdx.x];
+		else if (e < n)
+			w.x = pairs[e].x;
+		dg[2 * it] = e < n ? (w.x >> shift) & kDigitMask : (uint32_t)kBins;
+		dg[2 * it + 1] = e + 1 < n ? (w.z >> shift) & kDigitMask : (uint32_t)kBins;
 	}
 #pragma unroll
 	for (int it = 0; it < kSortItems; it++)
@@ -505,8 +512,10 @@ int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint2 **
 {
 	const int ntiles = sort_tiles(n);
 	const int64_t hlen = sort_hist_len(n);
-	if (reinterpret_cast<uintptr_t>(hist) % 16 != 0)  // k_scan_apply's vectors
-		return fail(XSB_ERR_INVALID, "sort histogram not 16-byte aligned");
+	// k_scan_apply's and k_hist_pairs' 16-byte vectors
+	if ((reinterpret_cast<uintptr_t>(hist) | reinterpret_cast<uintptr_t>(*p) |
+	     reinterpret_cast<uintptr_t>(*p2)) % 16 != 0)
+		return fail(XSB_ERR_INVALID, "sort buffers not 16-byte aligned");
 	int rc;
 	for (int pass = 0; pass < passes; pass++) {
 		const int shift = kDigitBits * pass;
@@ -551,13 +560,14 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	const int ntiles = sort_tiles(n);
 	const int64_t hlen = sort_hist_len(n);
 	const int nparts = sort_parts(n);
-	const size_t bytes = (size_t)n * 16 + (size_t)hlen * 4 + (size_t)nparts * 4 + 64 + 4 * kBins;
+	const int64_t n2 = (n + 1) & ~(int64_t)1;  // pair buffers start 16-byte aligned
+	const size_t bytes = (size_t)n2 * 16 + (size_t)hlen * 4 + (size_t)nparts * 4 + 64 + 4 * kBins;
 	void *buf = nullptr;
 	int rc = scratch_reserve(d, 0, bytes, &buf);
 	if (rc)
 		return rc;
-	uint2 *pA = (uint2 *)buf, *pB = pA + n;
-	uint32_t *hist = (uint32_t *)(pB + n), *part = hist + hlen, *starts = part + nparts + 16;
+	uint2 *pA = (uint2 *)buf, *pB = pA + n2;
+	uint32_t *hist = (uint32_t *)(pB + n2), *part = hist + hlen, *starts = part + nparts + 16;
 
 	hipLaunchKernelGGL(k_keys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, a->first_lookup, n,
 	                   a->seed, mode, pA, hist, ntiles);
