@@ -459,9 +459,11 @@ __global__ void k_interval_records(const double *__restrict__ grid, int64_t tota
 	}
 }
 
-// FP32 interval records (Tables::ivl32): {E_{k+1}, E_{k+1} - E_k,
+// FP32 interval records (Tables::ivl32): {E_{k+1}, 1 / (E_{k+1} - E_k),
 // float(xs_{k+1})[5], float(xs_{k+1}) - float(xs_k) [5], pad}; zero for the last
-// point of each nuclide.
+// point of each nuclide.  A reciprocal that is not a normal number (equal
+// energies, or a width above 2^1022) is stored as NaN: the lookup's error bound
+// is then NaN and the lookup is evaluated on the FP64 path.
 __global__ void k_interval_records32(const double *__restrict__ grid, int64_t total, int32_t n_gp,
                                      double *__restrict__ ivl)
 {
@@ -473,7 +475,8 @@ __global__ void k_interval_records32(const double *__restrict__ grid, int64_t to
 		double *r = ivl + i * kIvl32Doubles;
 		float *f = reinterpret_cast<float *>(r + 2);
 		r[0] = last ? 0.0 : lo[kPointDoubles];
-		r[1] = last ? 0.0 : lo[kPointDoubles] - lo[0];
+		const double rw = 1.0 / (lo[kPointDoubles] - lo[0]);
+		r[1] = last ? 0.0 : (__builtin_isnormal(rw) ? rw : __builtin_nan(""));
 #pragma unroll
 		for (int c = 0; c < 5; c++) {
 			const float xh = last ? 0.f : (float)lo[kPointDoubles + 1 + c];

@@ -13,7 +13,8 @@
 // roundoff of the table format (2^-24 FP32, 2^-8 BF16), u32 = 2^-24, M_j an
 // upper bound of |xs| for nuclide j and f_j the FP64 fraction, every term
 // conc_j * xs_j of the reference differs from ours by at most
-// conc_j M_j (1 + 2|f_j|) (u + 7 u32 + 5 u64), and the Kahan sum adds at most
+// conc_j M_j (1 + 2|f_j|) (u + 7 u32 + 8 u64) (5 u64 with the quotient as
+// fraction, 3 more with the interval records' reciprocal), and the Kahan sum adds at most
 // 2 u32 |S| + n u32^2 sum|t|; the reference's own ordered FP64 sum differs from
 // the exact sum by (n-1) u64 sum|r|.  So |S_c - R_c| <= B with
 //   B = 1.01 * A * (u + 12 u32),   A = sum_j conc_j M_j (1 + 2|f_j|)
@@ -104,21 +105,67 @@ __device__ __forceinline__ void mixed_term(double e_lo, const float xl[5], doubl
 	A += cb * (1.0f + 2.0f * fabsf(fr));
 }
 
-// mixed_term from an FP32 interval record: the same operations on the same
-// values (the FP64 width and the FP32 differences were rounded once when the
-// record was built, exactly as mixed_term rounds them).
+// One nuclide's term from an FP32 interval record R, added to the Kahan sums.
+// The FP32 differences were rounded once when the record was built, exactly
+// as mixed_term rounds them; the FP64 fraction is the numerator times the
+// record's rounded reciprocal width, within 3 u64 (relative) of the quotient
+// mixed_term takes.  The Kahan step's corrected term y = x c - comp is formed
+// as (xh c - comp) - (fr c) d with two fused operations: before the Kahan
+// step's own rounding of y, the roundings of fr c and of xh c - comp err by at
+// most u32 c M (1 + 2|f|) + u32 |comp|, where mixed_term's roundings of x and
+// x c may reach 2 u32 c M (1 + 2|f|), so the bound above holds for both.
+__device__ __forceinline__ float ivl32_fraction(const double *__restrict__ R, double p)
+{
+	return (float)((R[0] - p) * R[1]);
+}
+
+__device__ __forceinline__ void ivl32_terms(const double *__restrict__ R, float frc, float c,
+                                            const float comp[5], float y[5])
+{
+	const float *f = reinterpret_cast<const float *>(R + 2);  // xh[5], d[5]
+#pragma unroll
+	for (int k = 0; k < 5; k++)
+		y[k] = __builtin_fmaf(-frc, f[5 + k], __builtin_fmaf(f[k], c, -comp[k]));
+}
+
+// The Kahan step for a corrected term y = t - comp already formed.
+__device__ __forceinline__ void kahan_add_corrected(float &sum, float &comp, float y)
+{
+	const float s = sum + y;
+	comp = (s - sum) - y;
+	sum = s;
+}
+
+// mixed_term on an FP32 interval record: the record shared by the wave (R),
+// or one of the two consecutive records Q, Q + kIvl32Doubles (the second
+// where sel).  With two, both terms are formed straight from the shared
+// (scalar) records and each lane keeps its own, which is cheaper than first
+// selecting the records' 30 dwords per lane at three instructions each.
 __device__ __forceinline__ void mixed_term_ivl(const double *__restrict__ R, double p, float c,
                                                float cb, float sum[5], float comp[5], float &A)
 {
-	const double fr64 = (R[0] - p) / R[1];
-	const float fr = (float)fr64;
-	const float *f = reinterpret_cast<const float *>(R + 2);  // xh[5], d[5]
+	const float fr = ivl32_fraction(R, p);
+	float y[5];
+	ivl32_terms(R, fr * c, c, comp, y);
 #pragma unroll
-	for (int k = 0; k < 5; k++) {
-		const float x = __builtin_fmaf(-fr, f[5 + k], f[k]);
-		kahan_add(sum[k], comp[k], x * c);
-	}
+	for (int k = 0; k < 5; k++)
+		kahan_add_corrected(sum[k], comp[k], y[k]);
 	A += cb * (1.0f + 2.0f * fabsf(fr));
+}
+
+__device__ __forceinline__ void mixed_term_ivl2(const double *__restrict__ Q, bool sel, double p,
+                                                float c, float cb, float sum[5], float comp[5],
+                                                float &A)
+{
+	const float fa = ivl32_fraction(Q, p);
+	const float fb = ivl32_fraction(Q + kIvl32Doubles, p);
+	float ya[5], yb[5];
+	ivl32_terms(Q, fa * c, c, comp, ya);
+	ivl32_terms(Q + kIvl32Doubles, fb * c, c, comp, yb);
+#pragma unroll
+	for (int k = 0; k < 5; k++)
+		kahan_add_corrected(sum[k], comp[k], sel ? yb[k] : ya[k]);
+	A += cb * (1.0f + 2.0f * fabsf(sel ? fb : fa));
 }
 
 // The decision on the mixed sums (see the bound above): returns the argmax
@@ -237,16 +284,8 @@ __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc,
 		} else if (SHARE3 && (__ballot(du > 1u) == 0ULL || __ballot(du + 1u > 1u) == 0ULL)) {
 			// two records shared; each lane takes its own
 			const bool back = __ballot(du > 1u) != 0ULL;
-			const double *Q = back ? R - kIvl32Doubles : R;
-			const uint32_t sel = back ? du + 1u : du;
-			double a[2 * kIvl32Doubles], r[kIvl32Doubles];
-#pragma unroll
-			for (int w = 0; w < 2 * kIvl32Doubles; w++)
-				a[w] = Q[w];
-#pragma unroll
-			for (int w = 0; w < kIvl32Doubles; w++)
-				r[w] = sel ? a[kIvl32Doubles + w] : a[w];
-			mixed_term_ivl(r, p, c, cb, sum, comp, A);
+			mixed_term_ivl2(back ? R - kIvl32Doubles : R, (back ? du + 1u : du) != 0u, p, c, cb,
+			                sum, comp, A);
 			__builtin_amdgcn_sched_barrier(0);
 		} else {
 			mixed_term_ivl(T.ivl32 + ((int64_t)nuc * n_gp + k) * kIvl32Doubles, p, c, cb, sum,
