@@ -80,9 +80,9 @@ __device__ __forceinline__ void load_xs(const double *rec, float x[5])
 	}
 }
 
-__device__ __forceinline__ void kahan_add(float &sum, float &comp, float t)
+// The Kahan step for a corrected term y = t - comp already formed.
+__device__ __forceinline__ void kahan_add_corrected(float &sum, float &comp, float y)
 {
-	const float y = t - comp;
 	const float s = sum + y;
 	comp = (s - sum) - y;
 	sum = s;
@@ -90,30 +90,31 @@ __device__ __forceinline__ void kahan_add(float &sum, float &comp, float t)
 
 // One nuclide's term: interpolate the 5 channels between (e_lo, xl) and
 // (e_hi, xh) with the FP64 fraction, add x * c to the Kahan sums, and c_bound
-// (1 + 2|f|) to the bound accumulator A.
+// (1 + 2|f|) to the bound accumulator A.  The Kahan step's corrected term
+// y = x c - comp is formed as (xh c - comp) - (fr c) d, d = xh - xl, with two
+// fused operations: before the Kahan step's own rounding of y, the roundings
+// of fr c and of xh c - comp err by at most u32 c M (1 + 2|f|) + u32 |comp|,
+// where rounding x and x c separately may err by 2 u32 c M (1 + 2|f|); the
+// bound above counts the latter.
 __device__ __forceinline__ void mixed_term(double e_lo, const float xl[5], double e_hi,
                                            const float xh[5], double p, float c, float cb,
                                            float sum[5], float comp[5], float &A)
 {
 	const double fr64 = (e_hi - p) / (e_hi - e_lo);
 	const float fr = (float)fr64;
+	const float frc = fr * c;
 #pragma unroll
-	for (int k = 0; k < 5; k++) {
-		const float x = __builtin_fmaf(-fr, xh[k] - xl[k], xh[k]);
-		kahan_add(sum[k], comp[k], x * c);
-	}
+	for (int k = 0; k < 5; k++)
+		kahan_add_corrected(sum[k], comp[k],
+		                    __builtin_fmaf(-frc, xh[k] - xl[k], __builtin_fmaf(xh[k], c, -comp[k])));
 	A += cb * (1.0f + 2.0f * fabsf(fr));
 }
 
-// One nuclide's term from an FP32 interval record R, added to the Kahan sums.
-// The FP32 differences were rounded once when the record was built, exactly
-// as mixed_term rounds them; the FP64 fraction is the numerator times the
-// record's rounded reciprocal width, within 3 u64 (relative) of the quotient
-// mixed_term takes.  The Kahan step's corrected term y = x c - comp is formed
-// as (xh c - comp) - (fr c) d with two fused operations: before the Kahan
-// step's own rounding of y, the roundings of fr c and of xh c - comp err by at
-// most u32 c M (1 + 2|f|) + u32 |comp|, where mixed_term's roundings of x and
-// x c may reach 2 u32 c M (1 + 2|f|), so the bound above holds for both.
+// One nuclide's term from an FP32 interval record R, added to the Kahan sums
+// with mixed_term's operations.  The FP32 differences were rounded once when
+// the record was built, exactly as mixed_term rounds them; the FP64 fraction
+// is the numerator times the record's rounded reciprocal width, within 3 u64
+// (relative) of the quotient mixed_term takes.
 __device__ __forceinline__ float ivl32_fraction(const double *__restrict__ R, double p)
 {
 	return (float)((R[0] - p) * R[1]);
@@ -126,14 +127,6 @@ __device__ __forceinline__ void ivl32_terms(const double *__restrict__ R, float 
 #pragma unroll
 	for (int k = 0; k < 5; k++)
 		y[k] = __builtin_fmaf(-frc, f[5 + k], __builtin_fmaf(f[k], c, -comp[k]));
-}
-
-// The Kahan step for a corrected term y = t - comp already formed.
-__device__ __forceinline__ void kahan_add_corrected(float &sum, float &comp, float y)
-{
-	const float s = sum + y;
-	comp = (s - sum) - y;
-	sum = s;
 }
 
 // mixed_term on an FP32 interval record: the record shared by the wave (R),
