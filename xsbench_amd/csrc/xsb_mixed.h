@@ -240,6 +240,11 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 		const int nuc = ml[j];
 		const double *row = mixed_row<PREC>(T, nuc);
 		const int32_t lower = nuclide_lower<GRID, REC>(T, row, nuc, mrow, j, idx, p);
+		if (PREC == XSB_XS_FP32 && GRID == kUnionized) {  // FP32 interval records
+			mixed_term_ivl(T.ivl32 + ((int64_t)nuc * T.n_gp + lower) * kIvl32Doubles, p, ceff[j],
+			               cbnd[j], sum, comp, A);
+			continue;
+		}
 		const double *lo = row + (int64_t)lower * REC;
 		const double *hi = lo + REC;
 		float xl[5], xh[5];
