@@ -199,7 +199,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): the lookup path is "
-              f"single-device here; see DESIGN.md for how it shards.", file=sys.stderr)
+              f"single-device here (across GPUs it would split the lookup ids into contiguous "
+              f"ranges and add the per-GPU verification sums).", file=sys.stderr)
         sys.exit(2)
 
     import xsbench_amd as X

@@ -1,6 +1,7 @@
 /*
  * xsb_oracle.c -- CPU restatement of the XSBench lookup path (test
- * infrastructure only: the checker, never the thing measured or shipped).
+ * infrastructure only: it checks the HIP product and is never the thing
+ * measured or shipped).
  *
  * Restates openmp-threading/{Simulation.c, GridInit.c, Materials.c,
  * XSutils.c} of the reference; every function cites the lines it follows.

@@ -1,7 +1,7 @@
 /*
  * xsb_oracle.h -- CPU restatement of XSBench's macroscopic cross-section
- * lookup, used ONLY as the checker for the HIP product (tests, smoke() and
- * the cpu_baseline leg of bench.py).  Nothing in the product links it.
+ * lookup, used ONLY to check the HIP product (tests, smoke()) and as the
+ * cpu_baseline leg of bench.py.  Nothing in the product links it.
  *
  * Every function follows openmp-threading in the reference; each definition
  * in xsb_oracle.c cites the file:line it restates.  The structs keep the
