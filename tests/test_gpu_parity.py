@@ -253,6 +253,41 @@ def test_tiny_grid_lookups_match_oracle(orc, grid, gp):
             assert same_bits(out["macro"], macro) and np.array_equal(out["argmax"], am), (grid, gp, k)
 
 
+def test_zero_width_intervals_match_oracle(orc):
+    """Intervals of zero width: one nuclide's every second energy is lowered onto
+    the one before it after the grids were built, so the index grid still
+    brackets lookups into them and the reference divides by zero (infinite
+    sums). The FP64 schedules give the oracle's values, and with FP32 tables,
+    whose interval records then hold a NaN reciprocal, those lookups are
+    re-evaluated in FP64 and return the same sums."""
+    inp = X.make_inputs("small", "unionized", "event")
+    sd = X.grid_init(inp)
+    g = np.ctypeslib.as_array(C.cast(sd.nuclide_grid, C.POINTER(C.c_double)),
+                              (inp.n_isotopes, inp.n_gridpoints, 6))
+    mats = np.ctypeslib.as_array(sd.mats, (12 * sd.max_num_nucs,)).reshape(12, -1)
+    nuc = int(mats[2, 0])  # a nuclide of the most frequent material
+    hi = g[nuc, 2::2, 0]
+    g[nuc, 2::2, 0] = g[nuc, 1:-1:2, 0][: hi.shape[0]]
+    oi, osd = as_oracle(inp, sd)
+    first, n = 1000, 1 << 14
+    E, m, idx, macro, am = orc.event_detail(oi, osd, np.arange(first, first + n))
+    bad = ~np.isfinite(macro).all(axis=1)
+    assert bad.sum() > 1000
+    print(f"[gpu-test] zero-width intervals: {int(bad.sum())} of {n} lookups with infinite sums",
+          flush=True)
+    with X.Device(inp, sd, build_fp32=True) as dev:
+        for k in (X.K_BASELINE, X.K_SPLIT, X.K_MAT_E_SORT):
+            _, out = dev.event(n, first, kernel_id=k, detail=True)
+            assert np.array_equal(out["macro"], macro, equal_nan=True), k
+            assert same_bits(out["macro"][~bad], macro[~bad]) and np.array_equal(out["argmax"], am), k
+            if k == X.K_BASELINE:  # the fused kernel is FP64 only
+                continue
+            _, mix = dev.event(n, first, kernel_id=k, precision=X.XS_FP32, detail=True)
+            assert np.array_equal(mix["argmax"], am), k
+            assert np.all(mix["fallback"][bad] == 1), k
+            assert np.array_equal(mix["macro"][bad], macro[bad], equal_nan=True), k
+
+
 # ---------------------------------------------------------------------------
 # per-material unionized index tables: uint16 brackets when n_gridpoints <=
 # 65536 (the default), int32 otherwise or on request (BUILD_IDX32)
