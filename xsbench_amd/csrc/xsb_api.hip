@@ -160,6 +160,23 @@ __global__ void k_sorted_check(const double *__restrict__ a, int64_t n, unsigned
 		atomicAdd(bad, (unsigned long long)b);
 }
 
+// Descents (or NaN) inside the nuclides' energy rows (points [nuc][k], stride
+// kPointDoubles): with none, a bracket search over a short range may count
+// the energies <= p instead of bisecting (the same index on a sorted row).
+__global__ void k_rows_sorted_check(const double *__restrict__ grid, int64_t n_iso, int64_t n_gp,
+                                    unsigned long long *bad)
+{
+	uint32_t b = 0;
+	const int64_t n = n_iso * n_gp;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n;
+	     i += (int64_t)gridDim.x * blockDim.x)
+		if ((i + 1) % n_gp != 0)
+			b += (grid[i * kPointDoubles] <= grid[(i + 1) * kPointDoubles]) ? 0u : 1u;  // NaN too
+	b = wave_sum_u32(b);
+	if ((threadIdx.x & 63) == 0 && b)
+		atomicAdd(bad, (unsigned long long)b);
+}
+
 static int validate_tables(const xsb_tables *t, uint32_t flags)
 {
 	const bool gen = (flags & XSB_BUILD_GRIDS) != 0;
@@ -697,7 +714,10 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 		                   d->d_acc + 61);
 		XSB_HIP(hipGetLastError());
 	}
-	unsigned long long chk[2] = {0, 0};
+	hipLaunchKernelGGL(k_rows_sorted_check, dim3(2048), dim3(256), 0, d->stream, d->d_grid,
+	                   n_iso, n_gp, d->d_acc + 62);
+	XSB_HIP(hipGetLastError());
+	unsigned long long chk[3] = {0, 0, 0};
 	XSB_HIP(hipMemcpyAsync(chk, d->d_acc + 60, sizeof(chk), hipMemcpyDeviceToHost, d->stream));
 	XSB_HIP(hipStreamSynchronize(d->stream));
 	bad = chk[0];
@@ -717,7 +737,7 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	T.n_union = n_union;
 	T.n_iso = (int32_t)n_iso;
 	T.idx_pitch = (int32_t)pitch;
-	T.variant = kVariantDefault;
+	T.variant = kVariantDefault | (chk[2] == 0 ? kVariantSortedRows : 0);
 	T.n_gp = (int32_t)n_gp;
 	T.hash_bins = t->hash_bins;
 	T.max_num_nucs = t->max_num_nucs;

@@ -20,6 +20,7 @@ enum GridType : int { kUnionized = 0, kNuclide = 1, kHash = 2 };
 
 // Lookup-path choices (all give identical results).
 constexpr int kVariantWave = 1;  // wave-uniform path for material-uniform waves (unionized FP64)
+constexpr int kVariantSortedRows = 2;  // every nuclide's energy row is non-decreasing
 constexpr int kVariantDefault = kVariantWave;
 
 // Device view of the tables.  Passed by value as a kernel argument.
@@ -505,17 +506,28 @@ constexpr int kIdxChunk = 8;
 // (Simulation.c:273-302), with the bin's bounds u_low / u_high given.  When
 // every lane of the wave shares the bin, the bounds and their energies are
 // wave-uniform (scalar loads) and only the short bisection is per lane.
+// With sorted rows and a short range, the bisection's result (the last index
+// in [u_low, u_high) whose energy is <= p) is u_low plus the number of
+// interior energies <= p: those are read with scalar loads (the range is the
+// wave's) and counted, with no chain of per-lane probes.
 template <int STRIDE>
 __device__ __forceinline__ int32_t hash_bracket(const double *__restrict__ E_row, int32_t u_low,
-                                                int32_t u_high, int32_t n_gp, double p)
+                                                int32_t u_high, int32_t n_gp, double p,
+                                                bool sorted_rows)
 {
+	constexpr int32_t kCountMax = 16;
 	const double e_low = E_row[(int64_t)u_low * STRIDE];
 	const double e_high = E_row[(int64_t)u_high * STRIDE];
 	if (p <= e_low)
 		return 0;
 	if (p >= e_high)
 		return n_gp - 1;
-	return grid_search_nuclide<STRIDE>(p, E_row, u_low, u_high);
+	if (!sorted_rows || u_high - u_low > kCountMax)
+		return grid_search_nuclide<STRIDE>(p, E_row, u_low, u_high);
+	int32_t lower = u_low;
+	for (int32_t i = u_low + 1; i < u_high; i++)
+		lower += (E_row[(int64_t)i * STRIDE] <= p) ? 1 : 0;
+	return lower;
 }
 
 // Walks material m's nuclides for a wave whose lanes share the material and
@@ -529,13 +541,14 @@ __device__ __forceinline__ void hash_wave_rows(const Tables &T, const double *__
 	const int32_t *__restrict__ ml = T.mats + m * T.max_num_nucs;
 	const int32_t *__restrict__ lo_row = T.index_grid + idx * T.idx_pitch;
 	const bool last = idx == T.hash_bins - 1;
+	const bool sorted_rows = (T.variant & kVariantSortedRows) != 0;
 #pragma unroll 4
 	for (int j = 0; j < nn; j++) {
 		const int nuc = ml[j];
 		const int32_t u_low = lo_row[nuc];
 		const int32_t u_high = last ? n_gp - 1 : lo_row[T.idx_pitch + nuc] + 1;
 		const double *E_row = base + (int64_t)nuc * n_gp * STRIDE;
-		f(j, nuc, hash_bracket<STRIDE>(E_row, u_low, u_high, n_gp, p));
+		f(j, nuc, hash_bracket<STRIDE>(E_row, u_low, u_high, n_gp, p, sorted_rows));
 	}
 }
 
