@@ -288,6 +288,31 @@ def test_zero_width_intervals_match_oracle(orc):
             assert np.array_equal(mix["macro"][bad], macro[bad], equal_nan=True), k
 
 
+def test_unsorted_rows_keep_the_bisection(orc):
+    """A nuclide row that is not sorted (neighbouring energies swapped after
+    the grids were built): upload finds the descents, and the hash grid's
+    wave path then bisects as the reference does instead of counting the
+    bin range's energies; every lookup matches the oracle bit for bit."""
+    # ~14 points per bin (within the counted range), and enough lookups that
+    # sorted waves share a bin (the wave path)
+    inp = X.make_inputs("small", "hash", "event", hash_bins=800)
+    sd = X.grid_init(inp)
+    g = np.ctypeslib.as_array(C.cast(sd.nuclide_grid, C.POINTER(C.c_double)),
+                              (inp.n_isotopes, inp.n_gridpoints, 6))
+    mats = np.ctypeslib.as_array(sd.mats, (12 * sd.max_num_nucs,)).reshape(12, -1)
+    for nuc in {int(mats[2, 0]), int(mats[0, 0])}:
+        e = g[nuc, :, 0].copy()
+        g[nuc, 1:-2:3, 0] = e[2:-1:3]
+        g[nuc, 2:-1:3, 0] = e[1:-2:3]
+    oi, osd = as_oracle(inp, sd)
+    first, n = 777, 1 << 19
+    E, m, idx, macro, am = orc.event_detail(oi, osd, np.arange(first, first + n))
+    with X.Device(inp, sd) as dev:
+        for k in (X.K_BASELINE, X.K_MAT_E_SORT):
+            _, out = dev.event(n, first, kernel_id=k, detail=True)
+            assert same_bits(out["macro"], macro) and np.array_equal(out["argmax"], am), k
+
+
 # ---------------------------------------------------------------------------
 # per-material unionized index tables: uint16 brackets when n_gridpoints <=
 # 65536 (the default), int32 otherwise or on request (BUILD_IDX32)
