@@ -211,7 +211,10 @@ __device__ __forceinline__ int32_t ueg_bracket(const Tables &T, const uint8_t *r
 
 // mrow: the material's unionized index row (ueg_row), j: the nuclide's
 // position in the material list; both used on the unionized grid only.
-template <int GRID, int STRIDE>
+// COUNT (hash grid): lanes whose bins differ (the history rounds) count a
+// short range instead of bisecting it; where a wave's lanes mostly share a
+// bin the bisection's probes coalesce and are cheaper.
+template <int GRID, int STRIDE, bool COUNT = false>
 __device__ __forceinline__ int32_t nuclide_lower(const Tables &T, const double *__restrict__ E_row,
                                                  int nuc, const uint8_t *__restrict__ mrow, int j,
                                                  int64_t idx, double p)
@@ -229,12 +232,28 @@ __device__ __forceinline__ int32_t nuclide_lower(const Tables &T, const double *
 		                           : T.index_grid[(idx + 1) * T.idx_pitch + nuc] + 1;
 		const double e_low = E_row[(int64_t)u_low * STRIDE];
 		const double e_high = E_row[(int64_t)u_high * STRIDE];
-		if (p <= e_low)
+		if (COUNT) {
+			// on sorted rows a range of at most kCount points is counted (its
+			// interior energies <= p, loaded together with the bounds) instead
+			// of bisected with a chain of dependent probes: the same index
+			constexpr int32_t kCount = 8;
+			int32_t cnt = 0;
+#pragma unroll
+			for (int32_t i = 1; i < kCount; i++) {
+				const double e = E_row[(int64_t)min(u_low + i, u_high) * STRIDE];  // in bounds
+				cnt += (u_low + i < u_high && e <= p) ? 1 : 0;
+			}
+			lower = (p <= e_low) ? 0 : (p >= e_high) ? n_gp - 1 : u_low + cnt;
+			if (!((T.variant & kVariantSortedRows) && u_high - u_low <= kCount) && e_low < p &&
+			    p < e_high)
+				lower = grid_search_nuclide<STRIDE>(p, E_row, u_low, u_high);
+		} else if (p <= e_low) {
 			lower = 0;
-		else if (p >= e_high)
+		} else if (p >= e_high) {
 			lower = n_gp - 1;
-		else
+		} else {
 			lower = grid_search_nuclide<STRIDE>(p, E_row, u_low, u_high);
+		}
 	}
 	if (lower == n_gp - 1)
 		lower--;
@@ -301,7 +320,7 @@ __device__ __forceinline__ void accumulate_ivl(const double *__restrict__ R, dou
 }
 
 // calculate_macro_xs (openmp-threading/Simulation.c:326-375), FP64.
-template <int GRID>
+template <int GRID, bool COUNT = false>
 __device__ __forceinline__ void macro_xs_fp64(const Tables &T, double p, int mat, int64_t idx,
                                               double macro[5])
 {
@@ -316,7 +335,7 @@ __device__ __forceinline__ void macro_xs_fp64(const Tables &T, double p, int mat
 		const int nuc = ml[j];
 		const double conc = cl[j];
 		const double *row = T.grid + (int64_t)nuc * T.n_gp * kPointDoubles;
-		const int32_t lower = nuclide_lower<GRID, kPointDoubles>(T, row, nuc, mrow, j, idx, p);
+		const int32_t lower = nuclide_lower<GRID, kPointDoubles, COUNT>(T, row, nuc, mrow, j, idx, p);
 		accumulate_fp64(row + (int64_t)lower * kPointDoubles, p, conc, macro);
 	}
 }

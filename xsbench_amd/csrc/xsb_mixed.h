@@ -239,7 +239,7 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 	for (int j = 0; j < nn; j++) {
 		const int nuc = ml[j];
 		const double *row = mixed_row<PREC>(T, nuc);
-		const int32_t lower = nuclide_lower<GRID, REC>(T, row, nuc, mrow, j, idx, p);
+		const int32_t lower = nuclide_lower<GRID, REC, HISTORY>(T, row, nuc, mrow, j, idx, p);
 		if (PREC == XSB_XS_FP32 && GRID == kUnionized) {  // FP32 interval records
 			mixed_term_ivl(T.ivl32 + ((int64_t)nuc * T.n_gp + lower) * kIvl32Doubles, p, ceff[j],
 			               cbnd[j], sum, comp, A);
@@ -253,7 +253,7 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 		mixed_term(lo[0], xl, hi[0], xh, p, ceff[j], cbnd[j], sum, comp, A);
 	}
 	return mixed_finish<PREC, HISTORY, false>(sum, A, macro, fell_back, bound, [&](double m64[5]) {
-		macro_xs_fp64<GRID>(T, p, mat, idx, m64);
+		macro_xs_fp64<GRID, HISTORY>(T, p, mat, idx, m64);
 	});
 }
 

@@ -75,7 +75,7 @@ __device__ __forceinline__ int sorted_lookup(const Tables &T, double p, int mat,
 		if (wave && __ballot(mat != m0) == 0ULL)
 			macro_ueg_wave<CHUNK, SHARE3>(T, p, m0, idx, macro);
 		else
-			macro_xs_fp64<GRID>(T, p, mat, idx, macro);
+			macro_xs_fp64<GRID, HISTORY>(T, p, mat, idx, macro);
 		return argmax5(macro);
 	} else if (GRID == kHash) {
 		// wave path when the lanes share the material and the hash bin
@@ -95,7 +95,7 @@ __device__ __forceinline__ int sorted_lookup(const Tables &T, double p, int mat,
 			                                                          bnd);
 	}
 	if (PREC == XSB_XS_FP64) {
-		macro_xs_fp64<GRID>(T, p, mat, idx, macro);
+		macro_xs_fp64<GRID, HISTORY>(T, p, mat, idx, macro);
 		return argmax5(macro);
 	}
 	return macro_xs_mixed<GRID, PREC, HISTORY>(T, p, mat, idx, macro, &fb, bnd);
