@@ -168,9 +168,11 @@ int build_grids_on_device(Device *d, const xsb_tables *t, int64_t pitch, std::ve
 	// work buffers: raw points, two key/id pairs, sort histogram
 	const size_t raw_bytes = (size_t)n * kPointDoubles * 8;
 	const size_t pair_bytes = (size_t)n * (8 + 4);
-	const size_t hist_bytes = (size_t)sort_hist_len(n) * 4 + (size_t)sort_parts(n) * 4 + 256;
+	const size_t hist_bytes = (size_t)sort_hist_len(n) * 4 + (size_t)sort_parts(n) * 4;
 	void *work = nullptr;
-	if ((rc = device_alloc(&work, raw_bytes + 2 * pair_bytes + hist_bytes + 1024, "grid build work")))
+	// the seven buffers below, each rounded up to 256 bytes by take()
+	if ((rc = device_alloc(&work, raw_bytes + 2 * pair_bytes + hist_bytes + 7 * 256,
+	                       "grid build work")))
 		return rc;
 	char *w = (char *)work;
 	auto take = [&w](size_t b) {

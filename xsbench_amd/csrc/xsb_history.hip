@@ -191,8 +191,9 @@ int run_history_rounds(Device *d, const xsb_history_args *a, xsb_result *r)
 	const int L = a->lookups_per_particle;
 	hipStream_t s = d->stream;
 	const bool outs = a->out_particle_sum || a->out_final_seed;
+	// the eight buffers below, each rounded up to 256 bytes by take()
 	const size_t bytes = (size_t)P * (8 + 4 + 16) + (size_t)sort_hist_len(P) * 4 +
-	                     (size_t)sort_parts(P) * 4 + (outs ? (size_t)P * 16 : 0) + 1024;
+	                     (size_t)sort_parts(P) * 4 + (outs ? (size_t)P * 16 : 0) + 8 * 256;
 	void *buf = nullptr;
 	int rc = scratch_reserve(d, 3, bytes, &buf);
 	if (rc)
