@@ -331,12 +331,22 @@ __device__ __forceinline__ void macro_xs_fp64(const Tables &T, double p, int mat
 	const int32_t *__restrict__ ml = T.mats + mat * T.max_num_nucs;
 	const double *__restrict__ cl = T.concs + mat * T.max_num_nucs;
 	const uint8_t *__restrict__ mrow = (GRID == kUnionized) ? ueg_row(T, mat, idx) : nullptr;
-	for (int j = 0; j < nn; j++) {
+	auto step = [&](int j) {
 		const int nuc = ml[j];
 		const double conc = cl[j];
 		const double *row = T.grid + (int64_t)nuc * T.n_gp * kPointDoubles;
 		const int32_t lower = nuclide_lower<GRID, kPointDoubles, COUNT>(T, row, nuc, mrow, j, idx, p);
 		accumulate_fp64(row + (int64_t)lower * kPointDoubles, p, conc, macro);
+	};
+	if constexpr (COUNT) {
+		// two nuclides per trip: their load chains overlap (the history
+		// rounds' lanes rarely share data, so each lane waits on its own)
+#pragma unroll 2
+		for (int j = 0; j < nn; j++)
+			step(j);
+	} else {
+		for (int j = 0; j < nn; j++)
+			step(j);
 	}
 }
 
