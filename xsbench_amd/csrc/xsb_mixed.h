@@ -236,14 +236,14 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 	float sum[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
 	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
 	float A = 0.f;
-	for (int j = 0; j < nn; j++) {
+	auto step = [&](int j) {
 		const int nuc = ml[j];
 		const double *row = mixed_row<PREC>(T, nuc);
 		const int32_t lower = nuclide_lower<GRID, REC, HISTORY>(T, row, nuc, mrow, j, idx, p);
 		if (PREC == XSB_XS_FP32 && GRID == kUnionized) {  // FP32 interval records
 			mixed_term_ivl(T.ivl32 + ((int64_t)nuc * T.n_gp + lower) * kIvl32Doubles, p, ceff[j],
 			               cbnd[j], sum, comp, A);
-			continue;
+			return;
 		}
 		const double *lo = row + (int64_t)lower * REC;
 		const double *hi = lo + REC;
@@ -251,6 +251,15 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 		load_xs<PREC>(lo, xl);
 		load_xs<PREC>(hi, xh);
 		mixed_term(lo[0], xl, hi[0], xh, p, ceff[j], cbnd[j], sum, comp, A);
+	};
+	if constexpr (HISTORY) {
+		// two nuclides per trip, as macro_xs_fp64 in the history kernels
+#pragma unroll 2
+		for (int j = 0; j < nn; j++)
+			step(j);
+	} else {
+		for (int j = 0; j < nn; j++)
+			step(j);
 	}
 	return mixed_finish<PREC, HISTORY, false>(sum, A, macro, fell_back, bound, [&](double m64[5]) {
 		macro_xs_fp64<GRID, HISTORY>(T, p, mat, idx, m64);
