@@ -220,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_history(Tables T, int64_t particles,
 			double macro[5];
 			int am;
 			if (PREC == XSB_XS_FP64) {
-				macro_xs_fp64<GRID, true>(T, p, mat, idx, macro);
+				macro_xs_fp64<GRID>(T, p, mat, idx, macro);
 				am = argmax5(macro);
 			} else {
 				uint32_t f1 = 0;
