@@ -193,7 +193,8 @@ typedef struct {
 	int64_t bytes_index;      /* index grid (rows padded to 16 bytes) */
 	int64_t bytes_midx;       /* per-material copies of the unionized index grid */
 	int64_t bytes_union;      /* unionized energies */
-	int64_t bytes_ivl;        /* FP64 interval records of the unionized wave path */
+	int64_t bytes_ivl;        /* interval records of the unionized grid: FP64, plus FP32 with
+	                             XSB_BUILD_FP32 */
 	int32_t cu_count;
 	int32_t xcd_count;
 	char arch[32];
