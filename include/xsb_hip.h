@@ -234,6 +234,25 @@ int xsb_hip_lookup(void *handle, int64_t n, const double *d_energy, const int32_
                    int32_t xs_precision, double *d_macro, int32_t *d_argmax,
                    unsigned long long *d_vsum, void *stream);
 
+/* openmp-threading/Simulation.c:326 calculate_macro_xs over n samples held in
+ * host memory: energy[n], mat[n] (each in [0, 12)) -> macro[n x 5];
+ * synchronous, on the handle's stream.  Backs the host export
+ * calculate_macro_xs (include/xsbench.h); xsb_hip_lookup is the
+ * asynchronous form over device memory. */
+int xsb_hip_macro_host(void *handle, int64_t n, const double *energy, const int32_t *mat,
+                       int32_t xs_precision, double *macro);
+
+/* openmp-threading/Simulation.c:241 calculate_micro_xs over n samples held in
+ * host memory, FP64: energy[n], nuc[n] -> xs[n x 5].  lo/hi carry what the
+ * reference reads from index_data at the sample's grid index idx: on the
+ * unionized grid lo[i] = index_data[idx * n_isotopes + nuc] (hi unused), on
+ * the hash grid lo[i] = u_low and hi[i] = u_high (Simulation.c:273-281), on
+ * the nuclide grid neither (both may be NULL).  Every lo/hi must lie in
+ * [0, n_gridpoints - 1].  Synchronous.  Backs the host export
+ * calculate_micro_xs. */
+int xsb_hip_micro_host(void *handle, int64_t n, const double *energy, const int32_t *nuc,
+                       const int32_t *lo, const int32_t *hi, double *xs);
+
 /* Many independent batches (see xsb_batched_args); no reference counterpart. */
 int xsb_hip_batched(void *handle, const xsb_batched_args *args, xsb_result *res);
 

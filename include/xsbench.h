@@ -2,10 +2,14 @@
  * xsbench.h -- host surface of the MI355X XSBench lookup library
  * (libxsbench.so).  Keeps the reference's C interface: the same constants,
  * the same struct layouts and the same function signatures as
- * openmp-threading/XSbench_header.h, so that code written against the
- * reference links against this library unchanged.  The lookup drivers
- * (run_*) execute on the GPU through libxsb_hip.so (include/xsb_hip.h);
- * there is no CPU implementation of the lookup path in this library.
+ * openmp-threading/XSbench_header.h:104-150, every one of which this library
+ * defines, so that code written against the reference (its Main.c included)
+ * links against this library unchanged.  The lookup drivers (run_*) and
+ * calculate_micro_xs / calculate_macro_xs execute on the GPU through
+ * libxsb_hip.so (include/xsb_hip.h); there is no CPU implementation of the
+ * lookup in this library.  grid_search, grid_search_nuclide, pick_mat and
+ * the LCG functions are the reference's host helpers, defined here with its
+ * semantics; the GPU path does not call them.
  */
 #ifndef XSBENCH_H
 #define XSBENCH_H
@@ -98,9 +102,24 @@ unsigned long long run_event_based_simulation(Inputs in, SimulationData SD, int 
 unsigned long long run_history_based_simulation(Inputs in, SimulationData SD, int mype);  /* Simulation.c:116 */
 unsigned long long run_event_based_simulation_optimization_1(Inputs in, SimulationData SD,
                                                              int mype);                   /* Simulation.c:698 */
-/* Sampling helpers used by grid construction (Simulation.c:462-499) */
-double LCG_random_double(uint64_t *seed);
-uint64_t fast_forward_LCG(uint64_t seed, uint64_t n);
+/* Per-lookup functions (Simulation.c:241-499).  calculate_micro_xs and
+ * calculate_macro_xs run on the GPU: on a copy of the tables staged by
+ * move_simulation_data_to_device for the same host nuclide grid (and, for
+ * calculate_macro_xs, the same material arrays), else on one uploaded at the
+ * first call and kept until release_device_memory; one call is one device
+ * round trip (xsb_hip_macro_host / xsb_hip_micro_host take many samples). */
+void calculate_micro_xs(double p_energy, int nuc, long n_isotopes, long n_gridpoints,
+                        double *egrid, int *index_data, NuclideGridPoint *nuclide_grids,
+                        long idx, double *xs_vector, int grid_type, int hash_bins); /* Simulation.c:241 */
+void calculate_macro_xs(double p_energy, int mat, long n_isotopes, long n_gridpoints,
+                        int *num_nucs, double *concs, double *egrid, int *index_data,
+                        NuclideGridPoint *nuclide_grids, int *mats, double *macro_xs_vector,
+                        int grid_type, int hash_bins, int max_num_nucs);        /* Simulation.c:326 */
+long grid_search(long n, double quarry, double *A);                             /* Simulation.c:380 */
+long grid_search_nuclide(long n, double quarry, NuclideGridPoint *A, long low, long high); /* :403 */
+int pick_mat(uint64_t *seed);                                                   /* Simulation.c:426 */
+double LCG_random_double(uint64_t *seed);                                       /* Simulation.c:462 */
+uint64_t fast_forward_LCG(uint64_t seed, uint64_t n);                           /* Simulation.c:472 */
 
 /* ---- GridInit.c / Materials.c / XSutils.c ---- */
 SimulationData grid_init_do_not_profile(Inputs in, int mype);     /* GridInit.c:3 */
@@ -139,6 +158,8 @@ unsigned long long run_batched_simulation(Inputs in, SimulationData SD, int n_ba
                                           double *out_done_ms);
 
 /* ---- options of this implementation (no reference counterpart) ---- */
+#define XSBENCH_SCHEDULE_ASKED 0 /* run_* follow in.kernel_id (-k) */
+#define XSBENCH_SCHEDULE_AUTO 1  /* run_* take the fastest exact schedule whatever -k says */
 typedef struct {
 	int xs_precision;  /* 0 FP64 (bit-exact), 1 FP32 tables, 2 BF16 tables */
 	int device;        /* HIP device ordinal */
@@ -147,6 +168,9 @@ typedef struct {
 	double last_lookup_ms;  /* out: GPU time of its lookup kernels */
 	long last_fallbacks;    /* out: lookups re-evaluated in FP64 */
 	int last_error;         /* out: 0 or an XSB_ERR_* code */
+	int schedule;           /* XSBENCH_SCHEDULE_*; initially AUTO when the environment
+	                           holds XSBENCH_SCHEDULE=auto, else ASKED */
+	int last_kernel_id;     /* out: the schedule (-k id) the last run_* call ran */
 } xsbench_options;
 xsbench_options *xsbench_get_options(void);
 /* parallel exact construction of the unionized index grid (returns 1 when

@@ -632,3 +632,125 @@ def test_program_flow_valid_checksum(capfd, args):
     out = capfd.readouterr().out
     assert rc == 0, out[-3000:]
     assert "(Valid)" in out
+
+
+# ---------------------------------------------------------------------------
+# the reference's per-lookup functions exported by libxsbench.so
+# ---------------------------------------------------------------------------
+def _host_views(inp, sd):
+    a = X.sd_arrays(inp, sd)
+    grid_ptr = C.cast(sd.nuclide_grid, C.c_void_p)
+    ueg_ptr = C.cast(sd.unionized_energy_array, C.c_void_p) if sd.length_unionized_energy_array else None
+    ig_ptr = C.cast(sd.index_grid, C.c_void_p) if sd.length_index_grid else None
+    return a, grid_ptr, ueg_ptr, ig_ptr
+
+
+def _grid_index(lib, inp, sd, p):
+    """the grid-level index the reference computes for energy p (Simulation.c:340-352)"""
+    if inp.grid_type == X.UNIONIZED:
+        return lib.grid_search(sd.length_unionized_energy_array, p,
+                               C.cast(sd.unionized_energy_array, C.c_void_p))
+    if inp.grid_type == X.HASH:
+        return int(p / (1.0 / inp.hash_bins))
+    return 0
+
+
+@pytest.mark.parametrize("name", ["small_unionized", "small_hash", "large_unionized", "large_hash",
+                                  "xl45k_hash"])
+def test_exported_micro_xs_matches_reference_vectors(name):
+    """calculate_micro_xs (host export, run on the GPU) against the reference's
+    per-nuclide micro XS, bit for bit, on every nuclide of the golden lookups"""
+    inp, sd, dev = data(name)
+    lib = X.host_lib()
+    lib.move_simulation_data_to_device(inp, 1, sd)
+    try:
+        g = golden_npz(f"micro_{name}.npz")
+        a, grid_ptr, ueg_ptr, ig_ptr = _host_views(inp, sd)
+        out = (C.c_double * 5)()
+        for k in range(len(g["ids"])):
+            p, mat = float(g["energy"][k]), int(g["mat"][k])
+            want = g["micro"][g["offsets"][k]:g["offsets"][k + 1]]
+            idx = _grid_index(lib, inp, sd, p)
+            for j in range(int(a["num_nucs"][mat])):
+                nuc = int(a["mats"][mat, j])
+                lib.calculate_micro_xs(p, nuc, inp.n_isotopes, inp.n_gridpoints, ueg_ptr, ig_ptr,
+                                       grid_ptr, idx, out, inp.grid_type, inp.hash_bins)
+                assert same_bits(np.array(out[:]), want[j]), (k, j)
+    finally:
+        lib.release_device_memory(sd)
+
+
+@pytest.mark.parametrize("name", list(DATA_CFGS))
+def test_exported_macro_xs_matches_reference_vectors(name):
+    """calculate_macro_xs (host export, run on the GPU) on the golden lookups,
+    one call per lookup, and xsb_hip_macro_host on all of them at once"""
+    inp, sd, dev = data(name)
+    lib = X.host_lib()
+    g = golden_npz(f"lookups_{name}.npz")
+    a, grid_ptr, ueg_ptr, ig_ptr = _host_views(inp, sd)
+    out = (C.c_double * 5)()
+    for k in range(64):
+        p, mat = float(g["energy"][k]), int(g["mat"][k])
+        lib.calculate_macro_xs(p, mat, inp.n_isotopes, inp.n_gridpoints, sd.num_nucs, sd.concs,
+                               ueg_ptr, ig_ptr, grid_ptr, sd.mats, out, inp.grid_type,
+                               inp.hash_bins, sd.max_num_nucs)
+        assert same_bits(np.array(out[:]), g["macro"][k]), k
+    lib.release_device_memory(sd)  # the copy the first call uploaded
+    assert same_bits(dev.macro_host(g["energy"], g["mat"]), g["macro"])
+    # the FP32 tables through the same entry point: within the certified bound
+    mix = dev.macro_host(g["energy"], g["mat"], precision=X.XS_FP32)
+    assert np.all(np.abs(mix - g["macro"]) <= 1e-5 * np.abs(g["macro"]))
+
+
+def test_micro_host_rejects_out_of_range_brackets():
+    inp, sd, dev = data("small_hash")
+    with pytest.raises(X.XsbError) as e:
+        dev.micro_host([0.5], [0], lo=[0], hi=[inp.n_gridpoints])
+    assert e.value.code == 1
+    with pytest.raises(X.XsbError) as e:
+        dev.micro_host([0.5], [inp.n_isotopes], lo=[0], hi=[1])
+    assert e.value.code == 1
+    with pytest.raises(X.XsbError) as e:
+        dev.macro_host([0.5], [12])
+    assert e.value.code == 1
+
+
+def test_history_kernel_ids_and_auto_schedule(capfd):
+    """history mode with -k 3 runs (the reference ignores -k there) with the
+    published hash; XSBENCH_SCHEDULE_AUTO sends -k 0 event runs to the sorted
+    schedule"""
+    rc = X.run_main(["-m", "history", "-s", "small", "-k", "3"])
+    out = capfd.readouterr().out
+    assert rc == 0 and "(Valid)" in out, out[-3000:]
+    opts = X.host_lib().xsbench_get_options().contents
+    assert opts.last_kernel_id == 6  # 500,000 particles: the automatic choice takes rounds
+    lib = X.host_lib()
+    inp = X.make_inputs("small", "unionized", "event")
+    sd = X.grid_init(inp)
+    lib.move_simulation_data_to_device(inp, 1, sd)
+    try:
+        saved = opts.schedule
+        opts.schedule = X.SCHEDULE_AUTO
+        assert lib.run_event_based_simulation(inp, sd, 1) % 999983 == X.PUBLISHED[("event", "small")]
+        assert opts.last_kernel_id == X.K_MAT_E_SORT
+        opts.schedule = X.SCHEDULE_ASKED
+        assert lib.run_event_based_simulation(inp, sd, 1) % 999983 == X.PUBLISHED[("event", "small")]
+        assert opts.last_kernel_id == X.K_BASELINE
+        opts.schedule = saved
+    finally:
+        lib.release_device_memory(sd)
+
+
+def test_device_grid_build_many_nuclides_few_points():
+    """many nuclides on a two-point grid: the per-nuclide maxima of the device
+    build (n_iso doubles) exceed the build's sort histogram, so they need
+    their own buffer; tables equal the host build"""
+    inp = X.make_inputs("small", "hash", "event", gridpoints=2, hash_bins=64)
+    inp.n_isotopes = 1000
+    sd = X.grid_init(inp)
+    with X.Device(inp, sd) as host_dev, X.Device(inp, None) as gen_dev:
+        a, b = host_dev.download(), gen_dev.download()
+        for key in ("nuclide_grid", "index_grid"):
+            assert np.array_equal(a[key], b[key]), key
+        assert gen_dev.event(50_000, kernel_id=X.K_MAT_E_SORT)["vsum"] == \
+            host_dev.event(50_000, kernel_id=X.K_BASELINE)["vsum"]

@@ -204,19 +204,21 @@ def test_hip_library_exports_every_declared_symbol():
 _ABI_STRUCTS = {"xsb_tables": "XsbTables", "xsb_event_args": "XsbEventArgs",
                 "xsb_history_args": "XsbHistoryArgs", "xsb_batch_plan": "XsbBatchPlan",
                 "xsb_batched_args": "XsbBatchedArgs", "xsb_result": "XsbResult",
-                "xsb_info": "XsbInfo"}
+                "xsb_info": "XsbInfo", "xsbench_options": "XsbenchOptions"}
 _ABI_CONSTS = {"XSB_XS_FP64": "XS_FP64", "XSB_XS_FP32": "XS_FP32", "XSB_XS_BF16": "XS_BF16",
                "XSB_BUILD_FP32": "BUILD_FP32", "XSB_BUILD_BF16": "BUILD_BF16",
                "XSB_BUILD_GRIDS": "BUILD_GRIDS", "XSB_BUILD_IDX32": "BUILD_IDX32",
                "XSB_UNIONIZED": "UNIONIZED", "XSB_NUCLIDE": "NUCLIDE", "XSB_HASH": "HASH",
                "XSB_K_BASELINE": "K_BASELINE", "XSB_K_MAT_E_SORT": "K_MAT_E_SORT",
-               "XSB_K_AUTO": "K_AUTO"}
+               "XSB_K_AUTO": "K_AUTO", "XSBENCH_SCHEDULE_ASKED": "SCHEDULE_ASKED",
+               "XSBENCH_SCHEDULE_AUTO": "SCHEDULE_AUTO"}
 
 
 def test_abi_structs_and_constants_match_the_python_mirror(tmp_path):
     """sizes and field offsets of every xsb_hip.h struct, and its constants,
     as the C compiler sees them, against the ctypes mirror"""
-    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "xsb_hip.h"', "int main(void) {"]
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "xsb_hip.h"',
+             '#include "xsbench.h"', "int main(void) {"]
     for cs, ps in _ABI_STRUCTS.items():
         lines.append(f'printf("{cs} size %zu\\n", sizeof({cs}));')
         for f, _ in getattr(X, ps)._fields_:
@@ -344,3 +346,87 @@ def test_chunked_index_sweep_equals_serial_sweep():
         U = np.sort(E.ravel())
         R = int(rng.integers(1, 6))
         assert np.array_equal(_chunked_sweep(E, U, R), _serial_sweep(E, U)), (E, R)
+
+
+# ---------------------------------------------------------------------------
+# the reference's own program against this library
+# ---------------------------------------------------------------------------
+REF_OMP = "/root/reference/openmp-threading"
+LIBDIR = os.path.join(ROOT, "xsbench_amd", "lib")
+
+
+def _ref_prototypes():
+    """names of the functions openmp-threading/XSbench_header.h:104-150 declares"""
+    text = open(os.path.join(REF_OMP, "XSbench_header.h")).read()
+    body = text[text.index("// io.c"):]
+    return sorted(set(re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", body, re.M)))
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_OMP), reason="reference sources not mounted")
+def test_every_reference_header_function_links_from_the_library(tmp_path):
+    """a translation unit compiled against the reference's own header takes
+    the address of every function it declares; it links against
+    libxsbench.so alone (no reference object), so each one is defined there
+    with C linkage"""
+    names = _ref_prototypes()
+    for n in ("calculate_micro_xs", "calculate_macro_xs", "grid_search", "grid_search_nuclide",
+              "pick_mat", "run_event_based_simulation", "grid_init_do_not_profile"):
+        assert n in names
+    src = tmp_path / "all.c"
+    src.write_text('#include "XSbench_header.h"\n'
+                   "void *all_functions[] = {" + ", ".join(f"(void *){n}" for n in names) + "};\n"
+                   "int main(void) { return all_functions[0] == 0; }\n")
+    exe = tmp_path / "all"
+    subprocess.run(["gcc", "-std=gnu99", "-I", REF_OMP, str(src), "-o", str(exe), "-L", LIBDIR,
+                    "-lxsbench", "-lxsb_hip", f"-Wl,-rpath,{LIBDIR}", "-lm"], check=True)
+    assert subprocess.run([str(exe)], timeout=60).returncode == 0
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_OMP), reason="reference sources not mounted")
+def test_reference_main_links_and_reaches_the_device(tmp_path):
+    """openmp-threading/Main.c, unchanged, built against libxsbench.so: it
+    runs the reference's flow (CLI, grid init, the event driver) and, with no
+    GPU, stops cleanly at the upload with the library's message"""
+    exe = tmp_path / "XSBench_ref_main"
+    subprocess.run(["gcc", "-std=gnu99", "-O2", "-fopenmp", "-DOPENMP", "-I", REF_OMP,
+                    os.path.join(REF_OMP, "Main.c"), "-o", str(exe), "-L", LIBDIR, "-lxsbench",
+                    "-lxsb_hip", f"-Wl,-rpath,{LIBDIR}", "-lm"], check=True)
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1")
+    for k in ("0", "1"):
+        r = subprocess.run([str(exe), "-s", "small", "-m", "event", "-l", "1000", "-k", k,
+                            "-t", "2"], capture_output=True, text=True, timeout=300, env=env)
+        assert r.returncode == 1, r.stdout + r.stderr
+        assert "Intializing material data" in r.stdout or "Initializing" in r.stdout, r.stdout
+        assert "no usable HIP device" in r.stderr, r.stderr
+
+
+def test_host_search_and_pick_mat_helpers(orc):
+    """grid_search / grid_search_nuclide / pick_mat exported with the
+    reference's signatures: pick_mat against the reference's own outputs,
+    the searches against a brute-force clamped upper bound (ties, keys out
+    of range) and against the oracle"""
+    import bisect
+    lib = X.host_lib()
+    g = golden_npz("sampling.npz")
+    for s, m in zip(g["seeds"], g["pick_mat"]):
+        x = C.c_uint64(int(s))
+        y = C.c_uint64(int(s))
+        assert lib.pick_mat(C.byref(x)) == m
+        orc.pick_mat(C.byref(y))
+        assert x.value == y.value
+    rng = np.random.default_rng(11)
+    for n in (2, 3, 5, 16, 40):
+        for _ in range(25):
+            A = np.sort(rng.integers(0, 7, size=n).astype(np.float64) / 4.0)
+            pts = (O.NuclideGridPoint * n)()
+            for k in range(n):
+                pts[k].energy = A[k]
+            lo = int(rng.integers(0, n - 1))
+            hi = int(rng.integers(lo + 1, n))
+            for q in np.concatenate([A, [-1.0, 9.0, 0.3, 0.8]]):
+                want = min(max(bisect.bisect_right(list(A), q) - 1, 0), n - 2)
+                assert lib.grid_search(n, q, A.ctypes.data) == want
+                ub = bisect.bisect_right(list(A[lo:hi + 1]), q) + lo
+                want_n = min(max(ub - 1, lo), hi - 1) if hi - lo > 1 else lo
+                got = lib.grid_search_nuclide(n, q, C.addressof(pts), lo, hi)
+                assert got == want_n == orc.grid_search_nuclide(n, q, pts, lo, hi)

@@ -37,6 +37,8 @@ BUILD_FP32, BUILD_BF16, BUILD_GRIDS, BUILD_IDX32 = 0x1, 0x2, 0x4, 0x8
 # lookup schedules = the reference's -k kernel ids (cuda/Main.cu:62-83)
 K_BASELINE, K_SPLIT, K_MAT_LAUNCH, K_FUEL_LAUNCH, K_MAT_SORT, K_FUEL_SORT, K_MAT_E_SORT = range(7)
 K_AUTO = -1
+# xsbench_options.schedule (include/xsbench.h)
+SCHEDULE_ASKED, SCHEDULE_AUTO = 0, 1
 
 ERRORS = {0: "success", 1: "invalid argument", 2: "no usable HIP device", 3: "device out of memory",
           4: "HIP runtime error", 5: "unsupported option"}
@@ -150,6 +152,16 @@ class XsbResult(C.Structure):
                 "material_counts": [int(x) for x in self.material_counts]}
 
 
+class XsbenchOptions(C.Structure):
+    """xsbench_options (include/xsbench.h): options of the host surface"""
+    _fields_ = [
+        ("xs_precision", C.c_int), ("device", C.c_int), ("verbose", C.c_int),
+        ("last_kernel_ms", C.c_double), ("last_lookup_ms", C.c_double),
+        ("last_fallbacks", C.c_long), ("last_error", C.c_int), ("schedule", C.c_int),
+        ("last_kernel_id", C.c_int),
+    ]
+
+
 class XsbInfo(C.Structure):
     _fields_ = [
         ("device", C.c_int32), ("grid_type", C.c_int32), ("n_isotopes", C.c_int64),
@@ -165,14 +177,17 @@ class XsbInfo(C.Structure):
 # C-ABI entry points declared in include/xsb_hip.h (checked by the tests)
 HIP_SYMBOLS = ("xsb_hip_upload", "xsb_hip_release", "xsb_hip_event", "xsb_hip_history",
                "xsb_hip_lookup", "xsb_hip_batched", "xsb_hip_balg", "xsb_hip_copy_probe",
-               "xsb_hip_info", "xsb_hip_strerror", "xsb_hip_last_error", "xsb_hip_download")
+               "xsb_hip_info", "xsb_hip_strerror", "xsb_hip_last_error", "xsb_hip_download",
+               "xsb_hip_macro_host", "xsb_hip_micro_host")
 HOST_SYMBOLS = ("read_CLI", "print_inputs", "print_results", "grid_init_do_not_profile",
                 "load_num_nucs", "load_mats", "load_concs", "run_event_based_simulation",
                 "run_history_based_simulation", "run_event_based_simulation_optimization_1",
                 "move_simulation_data_to_device", "release_device_memory", "LCG_random_double",
                 "fast_forward_LCG", "binary_write", "binary_read", "estimate_mem_usage",
                 "xsbench_get_options", "xsbench_last_init_parallel", "xsbench_main",
-                "xsbench_make_batches", "xsbench_batch_sample", "run_batched_simulation")
+                "xsbench_make_batches", "xsbench_batch_sample", "run_batched_simulation",
+                "calculate_micro_xs", "calculate_macro_xs", "grid_search", "grid_search_nuclide",
+                "pick_mat")
 
 
 # ---------------------------------------------------------------------------
@@ -212,6 +227,8 @@ def hip_lib():
         _bind(L, "xsb_hip_copy_probe", C.c_int, [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)])
         _bind(L, "xsb_hip_info", C.c_int, [vp, C.POINTER(XsbInfo)])
         _bind(L, "xsb_hip_download", C.c_int, [vp, vp, vp, vp])
+        _bind(L, "xsb_hip_macro_host", C.c_int, [vp, C.c_int64, vp, vp, C.c_int32, vp])
+        _bind(L, "xsb_hip_micro_host", C.c_int, [vp, C.c_int64, vp, vp, vp, vp, vp])
         _bind(L, "xsb_hip_strerror", C.c_char_p, [C.c_int])
         _bind(L, "xsb_hip_last_error", C.c_char_p, [])
         _hip = L
@@ -236,8 +253,19 @@ def host_lib():
         _bind(L, "LCG_random_double", C.c_double, [C.POINTER(C.c_uint64)])
         _bind(L, "fast_forward_LCG", C.c_uint64, [C.c_uint64, C.c_uint64])
         _bind(L, "xsbench_last_init_parallel", C.c_int, [])
+        _bind(L, "xsbench_get_options", C.POINTER(XsbenchOptions), [])
         _bind(L, "estimate_mem_usage", C.c_size_t, [Inputs])
         _bind(L, "xsbench_main", C.c_int, [C.c_int, C.POINTER(C.c_char_p)])
+        _bind(L, "grid_search", C.c_long, [C.c_long, C.c_double, C.c_void_p])
+        _bind(L, "grid_search_nuclide", C.c_long, [C.c_long, C.c_double, C.c_void_p, C.c_long,
+                                                   C.c_long])
+        _bind(L, "pick_mat", C.c_int, [C.POINTER(C.c_uint64)])
+        _bind(L, "calculate_micro_xs", None,
+              [C.c_double, C.c_int, C.c_long, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p,
+               C.c_long, C.c_void_p, C.c_int, C.c_int])
+        _bind(L, "calculate_macro_xs", None,
+              [C.c_double, C.c_int, C.c_long, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p,
+               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int])
         _bind(L, "load_num_nucs", C.POINTER(C.c_int), [C.c_long])
         _bind(L, "load_mats", C.POINTER(C.c_int), [C.POINTER(C.c_int), C.c_long, C.POINTER(C.c_int)])
         _bind(L, "load_concs", C.POINTER(C.c_double), [C.POINTER(C.c_int), C.c_int])
@@ -509,6 +537,28 @@ class Device:
         ms = C.c_double()
         _check(self.lib.xsb_hip_copy_probe(self.handle, nbytes, reps, C.byref(ms)))
         return ms.value
+
+    def macro_host(self, energy, mat, precision=XS_FP64):
+        """calculate_macro_xs over host samples (xsb_hip_macro_host): [n, 5] sums."""
+        e = np.ascontiguousarray(energy, np.float64)
+        m = np.ascontiguousarray(mat, np.int32)
+        out = np.empty((len(e), 5), np.float64)
+        _check(self.lib.xsb_hip_macro_host(self.handle, len(e), e.ctypes.data, m.ctypes.data,
+                                           precision, out.ctypes.data))
+        return out
+
+    def micro_host(self, energy, nuc, lo=None, hi=None):
+        """calculate_micro_xs over host samples (xsb_hip_micro_host): [n, 5] XS."""
+        e = np.ascontiguousarray(energy, np.float64)
+        k = np.ascontiguousarray(nuc, np.int32)
+        lo = None if lo is None else np.ascontiguousarray(lo, np.int32)
+        hi = None if hi is None else np.ascontiguousarray(hi, np.int32)
+        out = np.empty((len(e), 5), np.float64)
+        _check(self.lib.xsb_hip_micro_host(self.handle, len(e), e.ctypes.data, k.ctypes.data,
+                                           None if lo is None else lo.ctypes.data,
+                                           None if hi is None else hi.ctypes.data,
+                                           out.ctypes.data))
+        return out
 
     def lookup(self, n, d_energy, d_mat, precision=XS_FP64, d_macro=None, d_argmax=None,
                d_vsum=None, stream=None):

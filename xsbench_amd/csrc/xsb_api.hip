@@ -1033,6 +1033,86 @@ int xsb_hip_lookup(void *handle, int64_t n, const double *d_energy, const int32_
 	                          d_vsum ? d_vsum : d->d_acc + 40, d->d_acc + 41, s);
 }
 
+int xsb_hip_macro_host(void *handle, int64_t n, const double *energy, const int32_t *mat,
+                       int32_t xs_precision, double *macro)
+{
+	Device *d;
+	int rc = check_handle(handle, &d);
+	if (rc)
+		return rc;
+	if (n < 0 || n > ((int64_t)1 << 32) || (n > 0 && (!energy || !mat || !macro)))
+		return fail(XSB_ERR_INVALID, "n=%lld or sample pointers invalid", (long long)n);
+	if ((rc = precision_ok(d, xs_precision)))
+		return rc;
+	for (int64_t i = 0; i < n; i++)
+		if ((uint32_t)mat[i] >= (uint32_t)kMaterials)
+			return fail(XSB_ERR_INVALID, "mat[%lld]=%d is not a material id", (long long)i, mat[i]);
+	if (n == 0)
+		return XSB_OK;
+	void *buf = nullptr;
+	if ((rc = scratch_reserve(d, 3, (size_t)n * (8 + 4 + 40) + 512, &buf)))
+		return rc;
+	double *dE = (double *)buf;
+	double *dM = dE + n;
+	int32_t *dmat = (int32_t *)(dM + 5 * n);
+	hipStream_t s = d->stream;
+	XSB_HIP(hipMemcpyAsync(dE, energy, (size_t)n * 8, hipMemcpyHostToDevice, s));
+	XSB_HIP(hipMemcpyAsync(dmat, mat, (size_t)n * 4, hipMemcpyHostToDevice, s));
+	DetailOut det{};
+	det.macro = dM;
+	if ((rc = launch_lookup_list(d, n, dE, dmat, nullptr, xs_precision, &det, d->d_acc + 40,
+	                             d->d_acc + 41, s)))
+		return rc;
+	XSB_HIP(hipMemcpyAsync(macro, dM, (size_t)n * 40, hipMemcpyDeviceToHost, s));
+	XSB_HIP(hipStreamSynchronize(s));
+	return XSB_OK;
+}
+
+int xsb_hip_micro_host(void *handle, int64_t n, const double *energy, const int32_t *nuc,
+                       const int32_t *lo, const int32_t *hi, double *xs)
+{
+	Device *d;
+	int rc = check_handle(handle, &d);
+	if (rc)
+		return rc;
+	const Tables &T = d->T;
+	const bool need_lo = T.grid_type != kNuclide, need_hi = T.grid_type == kHash;
+	if (n < 0 || n > ((int64_t)1 << 32) ||
+	    (n > 0 && (!energy || !nuc || !xs || (need_lo && !lo) || (need_hi && !hi))))
+		return fail(XSB_ERR_INVALID, "n=%lld or sample pointers invalid", (long long)n);
+	// every index the kernel reads with must lie in the nuclide's row
+	for (int64_t i = 0; i < n; i++) {
+		if (nuc[i] < 0 || nuc[i] >= T.n_iso)
+			return fail(XSB_ERR_INVALID, "nuc[%lld]=%d is not a nuclide id", (long long)i, nuc[i]);
+		if (need_lo && (lo[i] < 0 || lo[i] > T.n_gp - 1))
+			return fail(XSB_ERR_INVALID, "lo[%lld]=%d outside [0, %d]", (long long)i, lo[i],
+			            T.n_gp - 1);
+		if (need_hi && (hi[i] < 0 || hi[i] > T.n_gp - 1))
+			return fail(XSB_ERR_INVALID, "hi[%lld]=%d outside [0, %d]", (long long)i, hi[i],
+			            T.n_gp - 1);
+	}
+	if (n == 0)
+		return XSB_OK;
+	void *buf = nullptr;
+	if ((rc = scratch_reserve(d, 3, (size_t)n * (8 + 40 + 3 * 4) + 512, &buf)))
+		return rc;
+	double *dE = (double *)buf;
+	double *dX = dE + n;
+	int32_t *dnuc = (int32_t *)(dX + 5 * n), *dlo = dnuc + n, *dhi = dlo + n;
+	hipStream_t s = d->stream;
+	XSB_HIP(hipMemcpyAsync(dE, energy, (size_t)n * 8, hipMemcpyHostToDevice, s));
+	XSB_HIP(hipMemcpyAsync(dnuc, nuc, (size_t)n * 4, hipMemcpyHostToDevice, s));
+	if (need_lo)
+		XSB_HIP(hipMemcpyAsync(dlo, lo, (size_t)n * 4, hipMemcpyHostToDevice, s));
+	if (need_hi)
+		XSB_HIP(hipMemcpyAsync(dhi, hi, (size_t)n * 4, hipMemcpyHostToDevice, s));
+	if ((rc = launch_micro(d, n, dE, dnuc, dlo, dhi, dX, s)))
+		return rc;
+	XSB_HIP(hipMemcpyAsync(xs, dX, (size_t)n * 40, hipMemcpyDeviceToHost, s));
+	XSB_HIP(hipStreamSynchronize(s));
+	return XSB_OK;
+}
+
 int xsb_hip_balg(void *handle, const xsb_event_args *a, double *bytes_per_lookup)
 {
 	Device *d;

@@ -165,13 +165,15 @@ int build_grids_on_device(Device *d, const xsb_tables *t, int64_t pitch, std::ve
 	const int64_t n_iso = t->n_isotopes, n_gp = t->n_gridpoints, n = n_iso * n_gp;
 	const unsigned nb = 4096;
 	int rc;
-	// work buffers: raw points, two key/id pairs, sort histogram
+	// work buffers: raw points, two key/id pairs, sort histogram, the
+	// per-nuclide maxima
 	const size_t raw_bytes = (size_t)n * kPointDoubles * 8;
 	const size_t pair_bytes = (size_t)n * (8 + 4);
 	const size_t hist_bytes = (size_t)sort_hist_len(n) * 4 + (size_t)sort_parts(n) * 4;
+	const size_t max_bytes = (size_t)n_iso * 8;
 	void *work = nullptr;
-	// the seven buffers below, each rounded up to 256 bytes by take()
-	if ((rc = device_alloc(&work, raw_bytes + 2 * pair_bytes + hist_bytes + 7 * 256,
+	// the eight buffers below, each rounded up to 256 bytes by take()
+	if ((rc = device_alloc(&work, raw_bytes + 2 * pair_bytes + hist_bytes + max_bytes + 8 * 256,
 	                       "grid build work")))
 		return rc;
 	char *w = (char *)work;
@@ -185,6 +187,7 @@ int build_grids_on_device(Device *d, const xsb_tables *t, int64_t pitch, std::ve
 	uint32_t *vA = (uint32_t *)take((size_t)n * 4), *vB = (uint32_t *)take((size_t)n * 4);
 	uint32_t *hist = (uint32_t *)take((size_t)sort_hist_len(n) * 4);
 	uint32_t *part = (uint32_t *)take((size_t)sort_parts(n) * 4);
+	double *dmax = (double *)take(max_bytes);
 
 	auto done = [&](int code) {
 		(void)hipStreamSynchronize(s);
@@ -240,7 +243,6 @@ int build_grids_on_device(Device *d, const xsb_tables *t, int64_t pitch, std::ve
 			return done(fail(XSB_ERR_HIP, "index grid launch failed"));
 	}
 	// per-nuclide max |xs| for the compact tables' scales and bounds
-	double *dmax = (double *)hist;  // the histogram is free again
 	hipLaunchKernelGGL(k_nuclide_xs_max, dim3((unsigned)n_iso), dim3(256), 0, s, d->d_grid,
 	                   (int32_t)n_gp, dmax);
 	xs_max->assign((size_t)n_iso, 0.0);

@@ -98,6 +98,8 @@ int launch_sample(Device *d, int64_t first, int64_t n, uint64_t seed, double *E,
 int launch_lookup_list(Device *d, int64_t n, const double *E, const int32_t *mat,
                        const uint32_t *perm, int precision, const DetailOut *det,
                        unsigned long long *vsum, unsigned long long *nfallback, hipStream_t s);
+int launch_micro(Device *d, int64_t n, const double *E, const int32_t *nuc, const int32_t *lo,
+                 const int32_t *hi, double *xs, hipStream_t s);
 int launch_history(Device *d, int64_t particles, int lookups, uint64_t seed, int precision,
                    uint64_t *out_sum, uint64_t *out_seed, unsigned long long *vsum,
                    unsigned long long *nfallback, hipStream_t s);

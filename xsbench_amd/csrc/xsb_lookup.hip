@@ -199,6 +199,73 @@ int launch_lookup_list(Device *d, int64_t n, const double *E, const int32_t *mat
 }
 
 // ---------------------------------------------------------------------------
+// calculate_micro_xs (openmp-threading/Simulation.c:241-323) over explicit
+// samples: energy, nuclide, and the grid-level data the reference reads from
+// index_data at the sample's grid index -- the bracket on the unionized grid,
+// (u_low, u_high) on the hash grid; the nuclide grid bisects the whole row.
+// Bounds are checked on the host before the launch.
+// ---------------------------------------------------------------------------
+template <int GRID>
+__global__ __launch_bounds__(kBlock) void k_micro(Tables T, int64_t n, const double *__restrict__ E,
+                                                  const int32_t *__restrict__ nuc,
+                                                  const int32_t *__restrict__ lo,
+                                                  const int32_t *__restrict__ hi,
+                                                  double *__restrict__ xs)
+{
+	const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n)
+		return;
+	const double p = E[i];
+	const int32_t n_gp = T.n_gp;
+	const double *row = T.grid + (int64_t)nuc[i] * n_gp * kPointDoubles;
+	int32_t lower;
+	if (GRID == kNuclide) {
+		lower = grid_search_nuclide<kPointDoubles>(p, row, 0, n_gp - 1);
+	} else if (GRID == kUnionized) {
+		lower = lo[i];
+	} else {
+		const int32_t u_low = lo[i], u_high = hi[i];
+		if (p <= row[(int64_t)u_low * kPointDoubles])
+			lower = 0;
+		else if (p >= row[(int64_t)u_high * kPointDoubles])
+			lower = n_gp - 1;
+		else
+			lower = grid_search_nuclide<kPointDoubles>(p, row, u_low, u_high);
+	}
+	if (lower == n_gp - 1)
+		lower--;
+	const double *L = row + (int64_t)lower * kPointDoubles, *H = L + kPointDoubles;
+	const double f = (H[0] - p) / (H[0] - L[0]);
+#pragma unroll
+	for (int c = 0; c < 5; c++)
+		xs[5 * i + c] = H[1 + c] - f * (H[1 + c] - L[1 + c]);
+}
+
+int launch_micro(Device *d, int64_t n, const double *E, const int32_t *nuc, const int32_t *lo,
+                 const int32_t *hi, double *xs, hipStream_t s)
+{
+	if (n <= 0)
+		return XSB_OK;
+	const unsigned nb = blocks_for(n);
+	switch (d->T.grid_type) {
+	case kUnionized:
+		hipLaunchKernelGGL(k_micro<kUnionized>, dim3(nb), dim3(kBlock), 0, s, d->T, n, E, nuc, lo,
+		                   hi, xs);
+		break;
+	case kNuclide:
+		hipLaunchKernelGGL(k_micro<kNuclide>, dim3(nb), dim3(kBlock), 0, s, d->T, n, E, nuc, lo,
+		                   hi, xs);
+		break;
+	default:
+		hipLaunchKernelGGL(k_micro<kHash>, dim3(nb), dim3(kBlock), 0, s, d->T, n, E, nuc, lo, hi,
+		                   xs);
+		break;
+	}
+	XSB_HIP(hipGetLastError());
+	return XSB_OK;
+}
+
+// ---------------------------------------------------------------------------
 // History-based transport, one particle per lane
 // (openmp-threading/Simulation.c:141-237).
 // ---------------------------------------------------------------------------

@@ -46,19 +46,39 @@ uint64_t fast_forward_LCG(uint64_t seed, uint64_t n)
 	return (an * seed + cn) % LCG_M;
 }
 
-static xsbench_options g_opts = {0, 0, 0, 0.0, 0.0, 0, 0};
+static xsbench_options g_opts = {0, 0, 0, 0.0, 0.0, 0, 0, -1, 0};
+
+/* XSBENCH_SCHEDULE=auto in the environment selects the fastest exact
+ * schedule for every run_* call, so that a program built against the
+ * reference's own Main.c reaches it without a source change. */
+static void options_from_env(void)
+{
+	if (g_opts.schedule >= 0)
+		return;
+	const char *e = getenv("XSBENCH_SCHEDULE");
+	g_opts.schedule = (e && strcmp(e, "auto") == 0) ? XSBENCH_SCHEDULE_AUTO : XSBENCH_SCHEDULE_ASKED;
+}
 
 xsbench_options *xsbench_get_options(void)
 {
+	options_from_env();
 	return &g_opts;
 }
 
-/* ---- registry of device copies ---- */
+/* ---- registry of device copies ----
+ * An entry is keyed by the address of the host nuclide grid it was uploaded
+ * from, and remembers the material arrays it holds.  Entries come from
+ * move_simulation_data_to_device, or are made on first use by
+ * calculate_micro_xs / calculate_macro_xs (kept until release_device_memory
+ * of an SD with that nuclide grid, or the end of the process). */
 #define MAX_STAGED 16
 static struct {
 	const void *key;
 	void *handle;
 	int precision_flags;
+	const int *num_nucs;
+	const int *mats;
+	const double *concs;
 } g_staged[MAX_STAGED];
 static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
 
@@ -115,11 +135,37 @@ static void *find_staged(SimulationData SD)
 	pthread_mutex_lock(&g_lock);
 	for (int i = 0; i < MAX_STAGED; i++)
 		if (g_staged[i].key == SD.nuclide_grid && g_staged[i].handle &&
+		    g_staged[i].mats == SD.mats && g_staged[i].num_nucs == SD.num_nucs &&
+		    g_staged[i].concs == SD.concs &&
 		    (g_staged[i].precision_flags & flags_for_precision(g_opts.xs_precision)) ==
 		        flags_for_precision(g_opts.xs_precision))
 			h = g_staged[i].handle;
 	pthread_mutex_unlock(&g_lock);
 	return h;
+}
+
+/* Registers handle h for SD; exits when the registry is full. */
+static void register_staged(SimulationData SD, void *h, int precision_flags)
+{
+	pthread_mutex_lock(&g_lock);
+	int slot = -1;
+	for (int i = 0; i < MAX_STAGED && slot < 0; i++)
+		if (!g_staged[i].handle)
+			slot = i;
+	if (slot >= 0) {
+		g_staged[slot].key = SD.nuclide_grid;
+		g_staged[slot].handle = h;
+		g_staged[slot].precision_flags = precision_flags;
+		g_staged[slot].num_nucs = SD.num_nucs;
+		g_staged[slot].mats = SD.mats;
+		g_staged[slot].concs = SD.concs;
+	}
+	pthread_mutex_unlock(&g_lock);
+	if (slot < 0) {
+		xsb_hip_release(h);
+		fprintf(stderr, "XSBench: more than %d device copies staged\n", MAX_STAGED);
+		exit(1);
+	}
 }
 
 /* cuda/GridInit.cu:4-78 */
@@ -129,22 +175,7 @@ SimulationData move_simulation_data_to_device(Inputs in, int mype, SimulationDat
 		printf("Allocating and moving simulation data to GPU memory space...\n");
 	release_device_memory(SD);
 	void *h = upload(in, SD);
-	pthread_mutex_lock(&g_lock);
-	int slot = -1;
-	for (int i = 0; i < MAX_STAGED && slot < 0; i++)
-		if (!g_staged[i].handle)
-			slot = i;
-	if (slot >= 0) {
-		g_staged[slot].key = SD.nuclide_grid;
-		g_staged[slot].handle = h;
-		g_staged[slot].precision_flags = (int)flags_for_precision(g_opts.xs_precision);
-	}
-	pthread_mutex_unlock(&g_lock);
-	if (slot < 0) {
-		xsb_hip_release(h);
-		fprintf(stderr, "move_simulation_data_to_device: more than %d staged copies\n", MAX_STAGED);
-		exit(1);
-	}
+	register_staged(SD, h, (int)flags_for_precision(g_opts.xs_precision));
 	if (mype == 0) {
 		xsb_info info;
 		if (xsb_hip_info(h, &info) == XSB_OK)
@@ -191,6 +222,7 @@ static unsigned long long run_event(Inputs in, SimulationData SD, int kernel_id)
 	g_opts.last_kernel_ms = r.ms_total;
 	g_opts.last_lookup_ms = r.ms_lookup;
 	g_opts.last_fallbacks = r.n_fallback;
+	g_opts.last_kernel_id = r.kernel_id;
 	if (g_opts.verbose)
 		printf("GPU kernels: %.3f ms total, %.3f ms sampling/sort, %.3f ms lookups, %lld FP64 "
 		       "re-evaluations\n",
@@ -208,7 +240,8 @@ unsigned long long run_event_based_simulation(Inputs in, SimulationData SD, int 
 {
 	if (mype == 0)
 		printf("Beginning event based simulation...\n");
-	return run_event(in, SD, in.kernel_id);
+	options_from_env();
+	return run_event(in, SD, g_opts.schedule == XSBENCH_SCHEDULE_AUTO ? XSB_K_AUTO : in.kernel_id);
 }
 
 /* openmp-threading/Simulation.c:698-871: sample, sort by material and then
@@ -236,19 +269,172 @@ unsigned long long run_history_based_simulation(Inputs in, SimulationData SD, in
 	memset(&a, 0, sizeof(a));
 	a.particles = in.particles;
 	a.lookups_per_particle = in.lookups;
-	a.kernel_id = in.kernel_id;
+	/* the reference runs history mode whatever -k says (Main.c:80-93); here
+	 * -k 0 is one particle per lane and -k 6 the sorted rounds, and any other
+	 * id (or XSBENCH_SCHEDULE=auto) takes the automatic choice */
+	options_from_env();
+	a.kernel_id = (g_opts.schedule == XSBENCH_SCHEDULE_AUTO)       ? XSB_K_AUTO
+	              : (in.kernel_id == 0 || in.kernel_id == XSB_K_MAT_E_SORT) ? in.kernel_id
+	                                                                       : XSB_K_AUTO;
 	a.xs_precision = g_opts.xs_precision;
 	a.seed = STARTING_SEED;
 	xsb_result r;
 	int rc = xsb_hip_history(h, &a, &r);
 	if (rc)
 		die(rc, "xsb_hip_history");
+	g_opts.last_kernel_id = r.kernel_id;
 	g_opts.last_kernel_ms = r.ms_total;
 	g_opts.last_lookup_ms = r.ms_lookup;
 	g_opts.last_fallbacks = r.n_fallback;
 	if (temporary)
 		xsb_hip_release(h);
 	return r.vsum;
+}
+
+/* ---- per-lookup functions of the reference's interface ---- */
+
+/* openmp-threading/Simulation.c:380-400: the last index in [0, n - 1) whose
+ * value is <= quarry (clamped to [0, n - 2]), by bisection. */
+long grid_search(long n, double quarry, double *restrict A)
+{
+	long lo = 0, hi = n - 1;
+	while (hi - lo > 1) {
+		const long mid = lo + (hi - lo) / 2;
+		if (A[mid] > quarry)
+			hi = mid;
+		else
+			lo = mid;
+	}
+	return lo;
+}
+
+/* openmp-threading/Simulation.c:403-423: the same on A[low..high].energy. */
+long grid_search_nuclide(long n, double quarry, NuclideGridPoint *A, long low, long high)
+{
+	(void)n;
+	long lo = low, hi = high;
+	while (hi - lo > 1) {
+		const long mid = lo + (hi - lo) / 2;
+		if (A[mid].energy > quarry)
+			hi = mid;
+		else
+			lo = mid;
+	}
+	return lo;
+}
+
+/* openmp-threading/Simulation.c:426-460: one LCG draw against the volume
+ * fractions' reverse-order partial sums (dist[i] + dist[i-1] + ... + dist[1],
+ * summed in that order, so the thresholds are the reference's bit for bit);
+ * the first threshold above the roll names the material, else material 0. */
+int pick_mat(uint64_t *seed)
+{
+	static const double dist[12] = {0.140, 0.052, 0.275, 0.134, 0.154, 0.064,
+	                                0.066, 0.055, 0.008, 0.015, 0.025, 0.013};
+	const double roll = LCG_random_double(seed);
+	for (int i = 0; i < 12; i++) {
+		double running = 0.0;
+		for (int j = i; j > 0; j--)
+			running += dist[j];
+		if (roll < running)
+			return i;
+	}
+	return 0;
+}
+
+/* The device copy of the tables behind calculate_micro_xs/_macro_xs: a
+ * registered copy of the same host nuclide grid (with the same material
+ * arrays when `mats` is given), else one uploaded now and registered. */
+static void *lookup_tables(long n_isotopes, long n_gridpoints, int grid_type, int hash_bins,
+                           double *egrid, int *index_data, NuclideGridPoint *nuclide_grids,
+                           int *num_nucs, double *concs, int *mats, int max_num_nucs)
+{
+	void *h = NULL;
+	pthread_mutex_lock(&g_lock);
+	for (int i = 0; i < MAX_STAGED && !h; i++)
+		if (g_staged[i].handle && g_staged[i].key == nuclide_grids &&
+		    (!mats || (g_staged[i].mats == mats && g_staged[i].num_nucs == num_nucs &&
+		               g_staged[i].concs == concs)))
+			h = g_staged[i].handle;
+	pthread_mutex_unlock(&g_lock);
+	if (h)
+		return h;
+	/* calculate_micro_xs reads no material: one placeholder nuclide per material */
+	static int one_nuc[12] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+	static int nuc0[12] = {0};
+	static double conc0[12] = {0.0};
+	Inputs in;
+	memset(&in, 0, sizeof(in));
+	in.n_isotopes = n_isotopes;
+	in.n_gridpoints = n_gridpoints;
+	in.grid_type = grid_type;
+	in.hash_bins = hash_bins;
+	SimulationData SD;
+	memset(&SD, 0, sizeof(SD));
+	SD.nuclide_grid = nuclide_grids;
+	SD.length_nuclide_grid = (int)(n_isotopes * n_gridpoints);
+	SD.num_nucs = mats ? num_nucs : one_nuc;
+	SD.mats = mats ? mats : nuc0;
+	SD.concs = mats ? concs : conc0;
+	SD.max_num_nucs = mats ? max_num_nucs : 1;
+	if (grid_type == UNIONIZED) {
+		SD.unionized_energy_array = egrid;
+		SD.length_unionized_energy_array = (int)(n_isotopes * n_gridpoints);
+		SD.index_grid = index_data;
+		SD.length_index_grid = (long)SD.length_unionized_energy_array * n_isotopes;
+	} else if (grid_type == HASH) {
+		SD.index_grid = index_data;
+		SD.length_index_grid = (long)hash_bins * n_isotopes;
+	}
+	xsb_tables t;
+	fill_tables(in, SD, &t);
+	int rc = xsb_hip_upload(g_opts.device, &t, 0, &h);
+	if (rc)
+		die(rc, "xsb_hip_upload");
+	register_staged(SD, h, 0);
+	return h;
+}
+
+/* openmp-threading/Simulation.c:241-323, on the GPU: the host reads what the
+ * reference reads from index_data at grid index idx (the bracket, or the
+ * hash bin's bounds); the search and the interpolation run on the device
+ * copy of the tables (xsb_hip_micro_host).  One call is one device round
+ * trip: callers with many samples use xsb_hip_micro_host directly. */
+void calculate_micro_xs(double p_energy, int nuc, long n_isotopes, long n_gridpoints,
+                        double *restrict egrid, int *restrict index_data,
+                        NuclideGridPoint *restrict nuclide_grids, long idx,
+                        double *restrict xs_vector, int grid_type, int hash_bins)
+{
+	void *h = lookup_tables(n_isotopes, n_gridpoints, grid_type, hash_bins, egrid, index_data,
+	                        nuclide_grids, NULL, NULL, NULL, 0);
+	int32_t n32 = nuc, lo = 0, hi = 0;
+	if (grid_type == UNIONIZED) {
+		lo = index_data[idx * n_isotopes + nuc];
+	} else if (grid_type == HASH) {
+		lo = index_data[idx * n_isotopes + nuc];
+		hi = (idx == hash_bins - 1) ? (int32_t)(n_gridpoints - 1)
+		                            : index_data[(idx + 1) * n_isotopes + nuc] + 1;
+	}
+	int rc = xsb_hip_micro_host(h, 1, &p_energy, &n32, &lo, &hi, xs_vector);
+	if (rc)
+		die(rc, "xsb_hip_micro_host");
+}
+
+/* openmp-threading/Simulation.c:326-375, on the GPU (xsb_hip_macro_host,
+ * FP64: the reference's sums bit for bit).  One call is one device round
+ * trip: callers with many samples use xsb_hip_macro_host or xsb_hip_lookup. */
+void calculate_macro_xs(double p_energy, int mat, long n_isotopes, long n_gridpoints,
+                        int *restrict num_nucs, double *restrict concs, double *restrict egrid,
+                        int *restrict index_data, NuclideGridPoint *restrict nuclide_grids,
+                        int *restrict mats, double *restrict macro_xs_vector, int grid_type,
+                        int hash_bins, int max_num_nucs)
+{
+	void *h = lookup_tables(n_isotopes, n_gridpoints, grid_type, hash_bins, egrid, index_data,
+	                        nuclide_grids, num_nucs, concs, mats, max_num_nucs);
+	int32_t m32 = mat;
+	int rc = xsb_hip_macro_host(h, 1, &p_energy, &m32, XSB_XS_FP64, macro_xs_vector);
+	if (rc)
+		die(rc, "xsb_hip_macro_host");
 }
 
 /* ---- batched workload (no reference counterpart) ---- */
