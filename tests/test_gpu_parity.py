@@ -184,6 +184,18 @@ def test_history_xl45k_golden(golden_hashes, kernel):
         assert r["vsum"] == h["vsum"]
 
 
+@pytest.mark.parametrize("kernel", [0, 6])
+def test_history_xl45k_fp32_golden(golden_hashes, kernel):
+    """the XL-45000 hash-grid history workload (1M particles x 34 lookups) with
+    FP32 tables: the reference's own sum (its build in the survey container)"""
+    inp, sd, dev = data("xl45k_hash")
+    h = golden_hashes["history_xl45k_hash_1M"]
+    r = dev.history(h["particles"], h["lookups"], kernel_id=kernel, precision=X.XS_FP32)
+    print(f"[gpu-test] xl45k FP32 history 1M k={kernel}: {r['ms_total']:.2f} ms, "
+          f"{r['n_fallback']} FP64 re-evaluations", flush=True)
+    assert r["vsum"] == h["vsum"]
+
+
 def test_history_rounds_graph_replays():
     """the captured chain of rounds replays with the same result, with other
     work between replays, and recaptures for a new particle count"""
@@ -307,10 +319,21 @@ def test_unsorted_rows_keep_the_bisection(orc):
     oi, osd = as_oracle(inp, sd)
     first, n = 777, 1 << 19
     E, m, idx, macro, am = orc.event_detail(oi, osd, np.arange(first, first + n))
-    with X.Device(inp, sd) as dev:
+    # history: the rounds (and the mixed per-lane loop) count short bin
+    # ranges per lane on sorted rows; on these rows they must bisect too
+    P = 70_000
+    oh = O.Inputs.from_buffer_copy(oi)
+    oh.lookups = 34  # lookups per particle in history mode
+    sums, seeds = orc.history_detail(oh, osd, 0, P)
+    with X.Device(inp, sd, build_fp32=True) as dev:
         for k in (X.K_BASELINE, X.K_MAT_E_SORT):
             _, out = dev.event(n, first, kernel_id=k, detail=True)
             assert same_bits(out["macro"], macro) and np.array_equal(out["argmax"], am), k
+        for k in (0, 6):
+            for prec in (X.XS_FP64, X.XS_FP32):
+                _, h = dev.history(P, 34, kernel_id=k, precision=prec, detail=True)
+                assert np.array_equal(h["sums"], sums), (k, prec)
+                assert np.array_equal(h["final_seed"], seeds), (k, prec)
 
 
 # ---------------------------------------------------------------------------
@@ -498,6 +521,32 @@ def test_batched_per_batch_hashes(orc, mode, prec):
     if mode == 0:
         assert plan["sort"] in (1, 2) and plan["stage"] in (8, 16) and plan["tile"] >= 256
         assert plan["graph"] == 1
+
+
+_BATCH_TRUTH = {}
+
+
+@pytest.mark.parametrize("prec", [X.XS_FP64, X.XS_FP32])
+def test_batched_workload_large_against_oracle(orc, prec):
+    """the batched workload itself: 512 batches, sizes log-uniform in [1e3,
+    2e6] (122M lookups), on the large unionized grid with the autotuned plan;
+    every batch's sum against the oracle's (openmp-threading/Simulation.c:
+    326-375 per lookup, the batch's own stream and material CDF)"""
+    inp, sd, dev = data("large_unionized")
+    oi, osd = as_oracle(inp, sd)
+    sizes, seeds, thr = X.make_batches(2024, 512, 1000, 2_000_000)
+    if "want" not in _BATCH_TRUTH:
+        print(f"[gpu-test] oracle over {int(sizes.sum()):,} batched lookups", flush=True)
+        _BATCH_TRUTH["want"] = _batch_truth(orc, oi, osd, sizes, seeds, thr)
+    want = _BATCH_TRUTH["want"]
+    plan = {}
+    r, vs, done = dev.batched(sizes, seeds, thr, precision=prec, mode=0, plan=plan)
+    print(f"[gpu-test] batched large prec={prec}: {r['ms_total']:.2f} ms, plan {plan}", flush=True)
+    assert [int(v) for v in vs] == want
+    assert r["vsum"] == sum(want)
+    # the replayed graph of the tuned plan, with the 16-bit index tables it uses
+    r2, vs2, _ = dev.batched(sizes, seeds, thr, precision=prec, mode=0)
+    assert [int(v) for v in vs2] == want
 
 
 @pytest.mark.parametrize("name", ["small_unionized", "small_hash", "small_nuclide"])
