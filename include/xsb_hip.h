@@ -221,6 +221,19 @@ int xsb_hip_release(void *handle);
  * optimization_k, selected by args->kernel_id. */
 int xsb_hip_event(void *handle, const xsb_event_args *args, xsb_result *res);
 
+/* No counterpart in the reference: xsb_hip_event for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, schedules and results,
+ * but it allocates nothing: the sample and sort buffers come from one fixed
+ * array in the library per GPU (34.6 MB, held until the process ends,
+ * its address looked up by xsb_hip_upload), so at most
+ * XSB_NOALLOC_MAX_LOOKUPS lookups per call (larger counts return
+ * XSB_ERR_INVALID), and no per-lookup outputs (the out_* pointers must be
+ * NULL).  Every call on one GPU shares that array, from any handle on that
+ * GPU: calls that overlap on one GPU race. */
+#define XSB_NOALLOC_MAX_LOOKUPS (1 << 21)
+int xsb_hip_event_noalloc(void *handle, const xsb_event_args *args, xsb_result *res);
+
 /* openmp-threading/Simulation.c:116 run_history_based_simulation */
 int xsb_hip_history(void *handle, const xsb_history_args *args, xsb_result *res);
 

@@ -211,7 +211,8 @@ _ABI_CONSTS = {"XSB_XS_FP64": "XS_FP64", "XSB_XS_FP32": "XS_FP32", "XSB_XS_BF16"
                "XSB_UNIONIZED": "UNIONIZED", "XSB_NUCLIDE": "NUCLIDE", "XSB_HASH": "HASH",
                "XSB_K_BASELINE": "K_BASELINE", "XSB_K_MAT_E_SORT": "K_MAT_E_SORT",
                "XSB_K_AUTO": "K_AUTO", "XSBENCH_SCHEDULE_ASKED": "SCHEDULE_ASKED",
-               "XSBENCH_SCHEDULE_AUTO": "SCHEDULE_AUTO"}
+               "XSBENCH_SCHEDULE_AUTO": "SCHEDULE_AUTO",
+               "XSB_NOALLOC_MAX_LOOKUPS": "NOALLOC_MAX_LOOKUPS"}
 
 
 def test_abi_structs_and_constants_match_the_python_mirror(tmp_path):

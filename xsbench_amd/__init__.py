@@ -37,6 +37,7 @@ BUILD_FP32, BUILD_BF16, BUILD_GRIDS, BUILD_IDX32 = 0x1, 0x2, 0x4, 0x8
 # lookup schedules = the reference's -k kernel ids (cuda/Main.cu:62-83)
 K_BASELINE, K_SPLIT, K_MAT_LAUNCH, K_FUEL_LAUNCH, K_MAT_SORT, K_FUEL_SORT, K_MAT_E_SORT = range(7)
 K_AUTO = -1
+NOALLOC_MAX_LOOKUPS = 1 << 21  # XSB_NOALLOC_MAX_LOOKUPS
 # xsbench_options.schedule (include/xsbench.h)
 SCHEDULE_ASKED, SCHEDULE_AUTO = 0, 1
 
@@ -178,7 +179,7 @@ class XsbInfo(C.Structure):
 HIP_SYMBOLS = ("xsb_hip_upload", "xsb_hip_release", "xsb_hip_event", "xsb_hip_history",
                "xsb_hip_lookup", "xsb_hip_batched", "xsb_hip_balg", "xsb_hip_copy_probe",
                "xsb_hip_info", "xsb_hip_strerror", "xsb_hip_last_error", "xsb_hip_download",
-               "xsb_hip_macro_host", "xsb_hip_micro_host")
+               "xsb_hip_macro_host", "xsb_hip_micro_host", "xsb_hip_event_noalloc")
 HOST_SYMBOLS = ("read_CLI", "print_inputs", "print_results", "grid_init_do_not_profile",
                 "load_num_nucs", "load_mats", "load_concs", "run_event_based_simulation",
                 "run_history_based_simulation", "run_event_based_simulation_optimization_1",
@@ -221,6 +222,8 @@ def hip_lib():
         _bind(L, "xsb_hip_release", C.c_int, [vp])
         _bind(L, "xsb_hip_event", C.c_int, [vp, C.POINTER(XsbEventArgs), C.POINTER(XsbResult)])
         _bind(L, "xsb_hip_history", C.c_int, [vp, C.POINTER(XsbHistoryArgs), C.POINTER(XsbResult)])
+        _bind(L, "xsb_hip_event_noalloc", C.c_int,
+              [vp, C.POINTER(XsbEventArgs), C.POINTER(XsbResult)])
         _bind(L, "xsb_hip_lookup", C.c_int, [vp, C.c_int64, vp, vp, C.c_int32, vp, vp, vp, vp])
         _bind(L, "xsb_hip_batched", C.c_int, [vp, C.POINTER(XsbBatchedArgs), C.POINTER(XsbResult)])
         _bind(L, "xsb_hip_balg", C.c_int, [vp, C.POINTER(XsbEventArgs), C.POINTER(C.c_double)])
@@ -472,6 +475,20 @@ class Device:
         r = XsbResult()
         _check(self.lib.xsb_hip_event(self.handle, C.byref(a), C.byref(r)))
         return (r.as_dict(), out) if detail else r.as_dict()
+
+    def event_noalloc(self, lookups, first=0, kernel_id=K_BASELINE, precision=XS_FP64,
+                      seed=STARTING_SEED):
+        """xsb_hip_event_noalloc: the event run on the library's fixed work
+        memory (at most NOALLOC_MAX_LOOKUPS lookups, no per-lookup outputs)."""
+        a = XsbEventArgs()
+        a.lookups = lookups
+        a.first_lookup = first
+        a.seed = seed
+        a.kernel_id = kernel_id
+        a.xs_precision = precision
+        r = XsbResult()
+        _check(self.lib.xsb_hip_event_noalloc(self.handle, C.byref(a), C.byref(r)))
+        return r.as_dict()
 
     def history(self, particles, lookups=34, kernel_id=0, precision=XS_FP64, detail=False,
                 seed=STARTING_SEED):

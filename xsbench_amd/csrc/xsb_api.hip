@@ -5,10 +5,12 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "xsb_internal.h"
 #include "xsb_mixed.h"
+#include "xsb_sorted.h"
 
 namespace xsb {
 
@@ -617,6 +619,44 @@ static int build_material_index(Device *d, const xsb_tables *t, int64_t rows, in
 	return XSB_OK;
 }
 
+// ---- the allocation-free event entry point's work memory ----
+// One fixed array per GPU (each device holds its own copy of a __device__
+// variable), large enough for the sorted schedules' buffers at
+// XSB_NOALLOC_MAX_LOOKUPS lookups; held until the process ends.  In host code
+// `scratch` only stands in for the symbol: its device address comes from
+// hipGetSymbolAddress on each device, and only that address is used.
+constexpr size_t kNoallocBytes = (sorted_event_bytes(XSB_NOALLOC_MAX_LOOKUPS) + 255) & ~(size_t)255;
+static_assert(kNoallocBytes >= (size_t)XSB_NOALLOC_MAX_LOOKUPS * 12 + 256,
+              "the split schedule's sample arrays fit too");
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_lock;
+static void *g_noalloc_ptr[kMaxDevices];
+static int g_noalloc_rc[kMaxDevices];  // 0: not looked up yet, else an XSB_* code + 1
+
+// Looks up `scratch`'s address on `device` (current) once; called by
+// xsb_hip_upload, so the entry point itself makes no runtime call for it.
+static int noalloc_setup(int device)
+{
+	if (device < 0 || device >= kMaxDevices)
+		return fail(XSB_ERR_UNSUPPORTED, "device %d beyond the allocation-free table", device);
+	std::lock_guard<std::mutex> g(g_noalloc_lock);
+	if (g_noalloc_rc[device] == 0) {
+		void *ptr = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !ptr) {
+			(void)hipGetLastError();
+			g_noalloc_rc[device] = XSB_ERR_HIP + 1;
+			return fail(XSB_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s",
+			            device, hipGetErrorString(e));
+		}
+		g_noalloc_ptr[device] = ptr;
+		g_noalloc_rc[device] = XSB_OK + 1;
+	}
+	return g_noalloc_rc[device] - 1;
+}
+
 static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 {
 	int rc = validate_tables(t, flags);
@@ -633,6 +673,8 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	Device *d = new Device();
 	*out = d;
 	d->device = device;
+	if ((rc = noalloc_setup(device)))
+		return rc;
 	hipDeviceProp_t prop;
 	XSB_HIP(hipGetDeviceProperties(&prop, device));
 	d->cu_count = prop.multiProcessorCount;
@@ -859,7 +901,11 @@ static int detail_copy_back(Device *d, int64_t n, const xsb_event_args *a, const
 	return XSB_OK;
 }
 
-static int event_impl(Device *d, const xsb_event_args *a, xsb_result *r)
+// The event run.  Work buffers (sample arrays, sort buffers) come from
+// `work` (work_bytes, 256-byte aligned) when it is given, else from the
+// handle's scratch; per-lookup outputs always take the handle's scratch.
+static int event_impl(Device *d, const xsb_event_args *a, xsb_result *r, void *work = nullptr,
+                      size_t work_bytes = 0)
 {
 	if (!a || !r)
 		return fail(XSB_ERR_INVALID, "args and result must be non-NULL");
@@ -894,8 +940,11 @@ static int event_impl(Device *d, const xsb_event_args *a, xsb_result *r)
 			return rc;
 		XSB_HIP(hipEventRecord(d->ev[2], s));
 	} else if (k == XSB_K_SPLIT) {
-		void *buf = nullptr;
-		if ((rc = scratch_reserve(d, 0, (size_t)n * 12 + 256, &buf)))
+		void *buf = work;
+		if (work && work_bytes < (size_t)n * 12 + 256)
+			return fail(XSB_ERR_INVALID, "work buffer of %zu bytes for %lld lookups", work_bytes,
+			            (long long)n);
+		if (!work && (rc = scratch_reserve(d, 0, (size_t)n * 12 + 256, &buf)))
 			return rc;
 		double *E = (double *)buf;
 		int32_t *M = (int32_t *)((char *)buf + (size_t)n * 8);
@@ -907,7 +956,8 @@ static int event_impl(Device *d, const xsb_event_args *a, xsb_result *r)
 			return rc;
 		XSB_HIP(hipEventRecord(d->ev[2], s));
 	} else {
-		if ((rc = run_sorted_event(d, a, db.any ? &db.dev : nullptr, r, vsum, nfb)))
+		if ((rc = run_sorted_event(d, a, db.any ? &db.dev : nullptr, r, vsum, nfb, work,
+		                           work_bytes)))
 			return rc;
 	}
 	XSB_HIP(hipEventRecord(d->ev[3], s));
@@ -958,6 +1008,31 @@ int xsb_hip_event(void *handle, const xsb_event_args *args, xsb_result *res)
 	Device *d;
 	int rc = check_handle(handle, &d);
 	return rc ? rc : event_impl(d, args, res);
+}
+
+int xsb_hip_event_noalloc(void *handle, const xsb_event_args *args, xsb_result *res)
+{
+	Device *d;
+	int rc = check_handle(handle, &d);
+	if (rc)
+		return rc;
+	if (!args || !res)
+		return fail(XSB_ERR_INVALID, "args and result must be non-NULL");
+	if (args->out_energy || args->out_mat || args->out_index || args->out_macro ||
+	    args->out_argmax || args->out_bound || args->out_fallback)
+		return fail(XSB_ERR_INVALID, "xsb_hip_event_noalloc takes no per-lookup outputs");
+	if (args->lookups > XSB_NOALLOC_MAX_LOOKUPS)
+		return fail(XSB_ERR_INVALID, "lookups=%lld exceeds the allocation-free maximum %d",
+		            (long long)args->lookups, XSB_NOALLOC_MAX_LOOKUPS);
+	// set up by xsb_hip_upload on this device; only read here
+	void *work = nullptr;
+	{
+		std::lock_guard<std::mutex> g(g_noalloc_lock);
+		if (g_noalloc_rc[d->device] != XSB_OK + 1)
+			return fail(XSB_ERR_HIP, "no allocation-free work memory on device %d", d->device);
+		work = g_noalloc_ptr[d->device];
+	}
+	return event_impl(d, args, res, work, kNoallocBytes);
 }
 
 int xsb_hip_history(void *handle, const xsb_history_args *a, xsb_result *r)

@@ -110,9 +110,12 @@ int launch_copy(const void *src, void *dst, int64_t bytes, hipStream_t s);
 // ---- sorted schedules (xsb_sched.hip) ----
 // Enqueues the sorted schedule (no synchronisation; the digit starts are
 // copied to h_out + 2), then, once the stream has been synchronised,
-// finish_sorted_event fills r->material_counts from them.
+// finish_sorted_event fills r->material_counts from them.  The work buffers
+// (sorted_event_bytes(lookups), 16-byte aligned) come from `work` when it is
+// given, else from the handle's scratch.
 int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, xsb_result *r,
-                     unsigned long long *vsum, unsigned long long *nfb);
+                     unsigned long long *vsum, unsigned long long *nfb, void *work = nullptr,
+                     size_t work_bytes = 0);
 void finish_sorted_event(const Device *d, const xsb_event_args *a, xsb_result *r);
 
 // ---- grid construction on the device (xsb_gridinit.hip): allocates and

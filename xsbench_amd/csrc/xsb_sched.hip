@@ -543,7 +543,8 @@ int sort_pairs64(hipStream_t s, int64_t n, int passes, uint64_t **k, uint32_t **
 }
 
 int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, xsb_result *r,
-                     unsigned long long *vsum, unsigned long long *nfb)
+                     unsigned long long *vsum, unsigned long long *nfb, void *work,
+                     size_t work_bytes)
 {
 	const int64_t n = a->lookups;
 	hipStream_t s = d->stream;
@@ -561,11 +562,16 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	const int64_t hlen = sort_hist_len(n);
 	const int nparts = sort_parts(n);
 	const int64_t n2 = (n + 1) & ~(int64_t)1;  // pair buffers start 16-byte aligned
-	const size_t bytes = (size_t)n2 * 16 + (size_t)hlen * 4 + (size_t)nparts * 4 + 64 + 4 * kBins;
-	void *buf = nullptr;
-	int rc = scratch_reserve(d, 0, bytes, &buf);
-	if (rc)
+	const size_t bytes = sorted_event_bytes(n);
+	void *buf = work;
+	int rc;
+	if (work) {
+		if (work_bytes < bytes || reinterpret_cast<uintptr_t>(work) % 16 != 0)
+			return fail(XSB_ERR_INVALID, "work buffer of %zu bytes for %lld lookups (%zu needed)",
+			            work_bytes, (long long)n, bytes);
+	} else if ((rc = scratch_reserve(d, 0, bytes, &buf))) {
 		return rc;
+	}
 	uint2 *pA = (uint2 *)buf, *pB = pA + n2;
 	uint32_t *hist = (uint32_t *)(pB + n2), *part = hist + hlen, *starts = part + nparts + 16;
 

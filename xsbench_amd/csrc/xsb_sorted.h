@@ -27,6 +27,18 @@ __device__ __forceinline__ int tile_pos(int wave, int item, int lane)
 }
 
 inline int sort_tiles(int64_t n) { return (int)((n + kTile - 1) / kTile); }
+
+// Bytes of work memory the sorted event schedules take for n lookups: two
+// buffers of {key, id} pairs (16-byte aligned), the per-tile digit histogram,
+// the scan partials, and the last pass's digit starts.
+constexpr size_t sorted_event_bytes(int64_t n)
+{
+	return (size_t)((n + 1) & ~(int64_t)1) * 16 +
+	       (size_t)kBins * (size_t)((n + kTile - 1) / kTile) * 4 +
+	       (size_t)(((int64_t)kBins * ((n + kTile - 1) / kTile) + kScanChunk - 1) / kScanChunk) *
+	           4 +
+	       64 + 4 * kBins;
+}
 inline int64_t sort_hist_len(int64_t n) { return (int64_t)kBins * sort_tiles(n); }
 inline int sort_parts(int64_t n) { return (int)((sort_hist_len(n) + kScanChunk - 1) / kScanChunk); }
 
