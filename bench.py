@@ -30,18 +30,31 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profile(kernel_id):
-    """HBM bytes per lookup-kernel launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    if not os.path.exists(TRAFFIC_FILE):
+def traffic_from_profile(path=TRAFFIC_FILE):
+    """HBM bytes per lookup-kernel launch from a committed rocprofv3 --pmc
+    summary (read and write requests to HBM counted by request size)."""
+    if not os.path.exists(path):
         return None
     vals = {}
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         m = re.match(r"^\s*([\w.]+)\s*[:=]\s*([-+0-9.eE]+)", line)
         if m:
             vals[m.group(1)] = float(m.group(2))
-    key = f"k{kernel_id}.hbm_bytes_per_launch"
-    return vals.get(key)
+    return vals.get("hbm_bytes_per_launch")
+
+
+def roofline(compulsory_bytes, kernel_ms, traffic=None):
+    """HBM roofline of one lookup kernel: `achieved` is the launch's
+    compulsory bytes (the distinct 128-byte lines of device memory its
+    lookups read, xsb_hip_lines_*) over the kernel's time, so frac <= 1 short
+    of lines left in the last-level cache by earlier launches."""
+    achieved = compulsory_bytes / (kernel_ms * 1e-3) / 1e9
+    r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+         "compulsory_bytes": int(compulsory_bytes)}
+    if traffic:
+        r["traffic_over_compulsory"] = round(traffic / compulsory_bytes, 3)
+    return r
 
 
 def _cpu_model():
@@ -60,7 +73,8 @@ def cpu_baseline(inp, sd, runs=3):
     absent; all 17M lookups of the workload, median of `runs` runs, timed
     around the run_* call as the reference times it (sampling included)."""
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    nproc = len(os.sched_getaffinity(0))
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc, nproc)
     gomp = C.CDLL("libgomp.so.1")
     gomp.omp_set_num_threads(threads)
     oi = oracle.Inputs.from_buffer_copy(inp)
@@ -88,10 +102,11 @@ def cpu_baseline(inp, sd, runs=3):
             raise SystemExit(f"CPU baseline produced hash {v % 999983}, expected {HASH_LARGE}")
     dt = sorted(times)[len(times) // 2]
     return {"value": round(LOOKUPS / dt, 1), "unit": "lookups/s", "cores": threads, "kind": kind,
+            "nproc": nproc, "cpu_model": _cpu_model(),
             "sample": f"all {LOOKUPS:,} lookups of the workload, median of {runs} runs of "
                       f"run_event_based_simulation (openmp-threading, schedule(dynamic,100)) "
-                      f"on {threads} threads of {_cpu_model()}: "
-                      + ", ".join(f"{t:.2f}" for t in times) + " s"}
+                      f"on {threads} threads ({nproc} cores available to the process) of "
+                      f"{_cpu_model()}: " + ", ".join(f"{t:.2f}" for t in times) + " s"}
 
 
 def batched_variants(X, dev):
@@ -104,6 +119,7 @@ def batched_variants(X, dev):
     total = int(sizes.sum())
     out = {"batched_batches": len(sizes), "batched_lookups": total}
     ref_vs = None
+    lines = {}
     for name, prec, window in (("fp64", X.XS_FP64, 0), ("fp32", X.XS_FP32, 0),
                                ("fp32_window16M", X.XS_FP32, 16_000_000)):
         plan = {}
@@ -119,6 +135,10 @@ def batched_variants(X, dev):
         elif not np.array_equal(vs, ref_vs):
             raise SystemExit(f"batched {name}: per-batch sums differ from FP64")
         ms = r["ms_total"]
+        if prec not in lines:
+            lines[prec] = dev.lines_batched(sizes, seeds, thr, precision=prec)["bytes"]
+        out[f"batched_{name}_roofline_frac"] = roofline(lines[prec], r["ms_lookup"])["frac"]
+        out[f"batched_{name}_lookup_kernel_ms"] = round(r["ms_lookup"], 3)
         out[f"batched_{name}_lookups_per_s"] = round(total / (ms * 1e-3), 1)
         out[f"batched_{name}_ms"] = round(ms, 3)
         out[f"batched_{name}_sort_ms"] = round(r["ms_sample"], 3)
@@ -150,6 +170,9 @@ def hash_variants(X):
             if any(r["hash"] != HASH_LARGE for r in rr):
                 raise SystemExit(f"hash grid {name}: hash {rr[0]['hash']} != {HASH_LARGE}")
             ms = min(r["ms_total"] for r in rr)
+            kms = min(r["ms_lookup"] for r in rr)
+            out[f"hash_{name}_k6_roofline_frac"] = roofline(
+                dev.lines_event(LOOKUPS, kernel_id=6, precision=prec)["bytes"], kms)["frac"]
             out[f"hash_{name}_k6_lookups_per_s"] = round(LOOKUPS / (ms * 1e-3), 1)
             out[f"hash_{name}_k6_fp64_reevaluations"] = rr[0]["n_fallback"]
             _, det = dev.event(1_000_000, kernel_id=6, precision=prec, detail=True)
@@ -181,6 +204,9 @@ def history_variants(X):
             if any(r["vsum"] != want for r in rr):
                 raise SystemExit(f"history {name}: sum {rr[0]['vsum']} != {want}")
             ms = min(r["ms_total"] for r in rr)
+            if k == 6:  # the rounds' compulsory bytes over the whole run (keys, sorts, lookups)
+                out[f"history_xl45k_{name}_roofline_frac_whole_run"] = roofline(
+                    dev.lines_history(1_000_000, 34, precision=prec)["bytes"], ms)["frac"]
             out[f"history_xl45k_{name}_lookups_per_s"] = round(34e6 / (ms * 1e-3), 1)
             log(f"history XL-45000 hash {name}: {ms:.3f} ms per 34M lookups")
         out["history_xl45k_hash"] = want % 999983
@@ -222,11 +248,13 @@ def main():
 
     # pick the schedule: fastest exact FP64 schedule over two timed calls each
     candidates = [args.kernel] if args.kernel >= 0 else [0, 1, 4, 6]
-    best, best_ms, sched_ms = None, 1e30, {}
+    best, best_ms, sched_ms, sched_kernel_ms = None, 1e30, {}, {}
     for k in candidates:
         run(k)
-        ms = min(run(k)["ms_total"] for _ in range(2))
+        rr = [run(k) for _ in range(2)]
+        ms = min(r["ms_total"] for r in rr)
         sched_ms[k] = ms
+        sched_kernel_ms[k] = min(r["ms_lookup"] for r in rr)
         log(f"schedule k={k}: {ms:.3f} ms per 17M lookups")
         if ms < best_ms:
             best, best_ms = k, ms
@@ -246,15 +274,20 @@ def main():
         f"(lookup kernel {ms_lookup:.3f} ms)")
 
     balg = dev.balg(LOOKUPS, precision=X.XS_FP64)
-    achieved = balg * LOOKUPS / (ms_lookup * 1e-3) / 1e9
-    traffic = traffic_from_profile(k)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "algorithmic_bytes_per_lookup": round(balg, 2),
-                "kernel": "k_lookup_ids" if k >= 2 else ("k_lookup_list" if k == 1 else "k_event_fused"),
-                "kernel_ms": round(ms_lookup, 4)}
+    comp = dev.lines_event(LOOKUPS, kernel_id=k, precision=X.XS_FP64)
+    roof = roofline(comp["bytes"], ms_lookup, traffic_from_profile() if k == 6 else None)
+    roof.update({"algorithmic_bytes_per_lookup": round(balg, 2),
+                 "kernel": "k_lookup_ids" if k >= 2 else ("k_lookup_list" if k == 1 else "k_event_fused"),
+                 "kernel_ms": round(ms_lookup, 4)})
+    if roof["frac"] > 1.0:
+        raise SystemExit(f"roofline fraction {roof['frac']} above 1: compulsory bytes miscounted")
+    log(f"roofline: {comp['bytes'] / 1e9:.3f} GB compulsory per launch over {ms_lookup:.3f} ms = "
+        f"{roof['achieved']:.0f} GB/s ({roof['frac']:.3f} of {HBM_PEAK_GBS:.0f} GB/s)")
 
     variants = {f"fp64_k{kk}_lookups_per_s": round(LOOKUPS / (ms * 1e-3), 1) for kk, ms in sched_ms.items()}
+    for kk, kms in sched_kernel_ms.items():
+        variants[f"fp64_k{kk}_roofline_frac"] = roofline(
+            dev.lines_event(LOOKUPS, kernel_id=kk)["bytes"], kms)["frac"]
     if not args.no_variants:
         import numpy as np
         _, ref = dev.event(1_000_000, kernel_id=6, detail=True)
@@ -262,6 +295,9 @@ def main():
             run(6, prec)
             rr = [run(6, prec) for _ in range(3)]
             ms = min(r["ms_total"] for r in rr)
+            variants[f"{name}_k6_roofline_frac"] = roofline(
+                dev.lines_event(LOOKUPS, kernel_id=6, precision=prec)["bytes"],
+                min(r["ms_lookup"] for r in rr))["frac"]
             variants[f"{name}_k6_lookups_per_s"] = round(LOOKUPS / (ms * 1e-3), 1)
             variants[f"{name}_k6_fp64_reevaluations"] = rr[0]["n_fallback"]
             _, det = dev.event(1_000_000, kernel_id=6, precision=prec, detail=True)
@@ -274,6 +310,11 @@ def main():
         if "fp64_k0_lookups_per_s" in variants:
             variants["mixed_batched_fp32_vs_plain_fp64_k0"] = round(
                 variants["batched_fp32_lookups_per_s"] / variants["fp64_k0_lookups_per_s"], 2)
+        variants["mixed_batched_fp32_vs_batched_fp64"] = round(
+            variants["batched_fp32_lookups_per_s"] / variants["batched_fp64_lookups_per_s"], 3)
+        if "fp64_k6_lookups_per_s" in variants:
+            variants["fp32_k6_vs_fp64_k6"] = round(
+                variants["fp32_k6_lookups_per_s"] / variants["fp64_k6_lookups_per_s"], 3)
         variants.update(hash_variants(X))
         variants.update(history_variants(X))
 
@@ -302,7 +343,7 @@ def main():
                                "grid 5.7 GB), 17,000,000 lookups, FP64, hash 952131",
                    "lookups": LOOKUPS, "grid": "unionized", "kernel_id": k,
                    "gpu_ms_per_step": round(ms_total, 4)},
-        "roofline": roofline,
+        "roofline": roof,
         "cpu_baseline": cpu,
         "variants": variants,
     }

@@ -190,7 +190,8 @@ typedef struct {
 	int64_t bytes_fp64;       /* FP64 AoS grid */
 	int64_t bytes_fp32;       /* FP32 XS table (0 if not built) */
 	int64_t bytes_bf16;       /* BF16 XS table (0 if not built) */
-	int64_t bytes_index;      /* index grid (rows padded to 16 bytes) */
+	int64_t bytes_index;      /* hash index grid (rows padded to 16 bytes); the unionized index
+	                             grid is not kept whole on the device (0) */
 	int64_t bytes_midx;       /* per-material copies of the unionized index grid */
 	int64_t bytes_union;      /* unionized energies */
 	int64_t bytes_ivl;        /* interval records of the unionized grid: FP64, plus FP32 with
@@ -274,12 +275,47 @@ int xsb_hip_batched(void *handle, const xsb_batched_args *args, xsb_result *res)
  * the table format of `xs_precision`. */
 int xsb_hip_balg(void *handle, const xsb_event_args *args, double *bytes_per_lookup);
 
+/* Measurement helpers, no reference counterpart: the compulsory HBM bytes of
+ * a workload -- the distinct 128-byte lines of device memory its lookups read
+ * at least once (a line shared by many lookups counts once), in the layout
+ * of the handle's tables and the format of xs_precision, plus the buffers of
+ * the schedule that the lookup kernel reads and writes and the small tables.
+ * No kernel can move fewer bytes from HBM for the same launch on this layout,
+ * so bytes / lookup-kernel time / HBM peak is a roofline fraction of at most
+ * 1 (up to lines still resident in the last-level cache from earlier work).
+ * The FP64 re-evaluations of mixed-precision lookups are not counted. */
+#define XSB_LINE_REGIONS 9
+typedef struct {
+	int64_t bytes;           /* bytes_tables + bytes_schedule + bytes_small */
+	int64_t bytes_tables;    /* lines of the lookup tables (sum of region_bytes) */
+	int64_t bytes_schedule;  /* sort pairs / sample arrays / particle states the kernel touches */
+	int64_t bytes_small;     /* skip table, material tables and constants */
+	int64_t launches;        /* lookup-kernel launches the bytes spread over (history: rounds) */
+	/* per table: unionized energies, unionized search brackets, per-material
+	 * index tables, FP64 interval records, FP32 interval records, hash index
+	 * grid, FP64 points, FP32 records, BF16 records */
+	int64_t region_bytes[XSB_LINE_REGIONS];
+} xsb_lines;
+
+/* The event run of `args` (per-lookup outputs ignored): lookups of the
+ * sorted schedules' kernels for kernel_id >= 2, one lane per lookup below. */
+int xsb_hip_lines_event(void *handle, const xsb_event_args *args, xsb_lines *out);
+/* The history run of `args`, as its sorted rounds read it; one launch per round. */
+int xsb_hip_lines_history(void *handle, const xsb_history_args *args, xsb_lines *out);
+/* The batched run of `args` (sizes, seeds, thresholds, xs_precision). */
+int xsb_hip_lines_batched(void *handle, const xsb_batched_args *args, xsb_lines *out);
+
 /* Device copy bandwidth probe: copies `bytes` once per rep with a plain
  * vector copy kernel and returns the mean kernel time (for scaling HBM
  * counters). */
 int xsb_hip_copy_probe(void *handle, int64_t bytes, int32_t reps, double *ms_per_copy);
 
 int xsb_hip_info(void *handle, xsb_info *info);
+/* No counterpart: the table sizes (xsb_info's bytes_* fields) that
+ * xsb_hip_upload of t with `flags` would keep resident, computed on the host
+ * from the sizes and the material tables alone (no device needed); for
+ * checking that a problem fits one GPU before building it. */
+int xsb_hip_table_bytes(const xsb_tables *t, uint32_t flags, xsb_info *info);
 const char *xsb_hip_strerror(int code);
 /* message of the last failure on the calling thread ("" if none) */
 const char *xsb_hip_last_error(void);

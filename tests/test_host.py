@@ -204,7 +204,7 @@ def test_hip_library_exports_every_declared_symbol():
 _ABI_STRUCTS = {"xsb_tables": "XsbTables", "xsb_event_args": "XsbEventArgs",
                 "xsb_history_args": "XsbHistoryArgs", "xsb_batch_plan": "XsbBatchPlan",
                 "xsb_batched_args": "XsbBatchedArgs", "xsb_result": "XsbResult",
-                "xsb_info": "XsbInfo", "xsbench_options": "XsbenchOptions"}
+                "xsb_info": "XsbInfo", "xsbench_options": "XsbenchOptions", "xsb_lines": "XsbLines"}
 _ABI_CONSTS = {"XSB_XS_FP64": "XS_FP64", "XSB_XS_FP32": "XS_FP32", "XSB_XS_BF16": "XS_BF16",
                "XSB_BUILD_FP32": "BUILD_FP32", "XSB_BUILD_BF16": "BUILD_BF16",
                "XSB_BUILD_GRIDS": "BUILD_GRIDS", "XSB_BUILD_IDX32": "BUILD_IDX32",
@@ -212,7 +212,7 @@ _ABI_CONSTS = {"XSB_XS_FP64": "XS_FP64", "XSB_XS_FP32": "XS_FP32", "XSB_XS_BF16"
                "XSB_K_BASELINE": "K_BASELINE", "XSB_K_MAT_E_SORT": "K_MAT_E_SORT",
                "XSB_K_AUTO": "K_AUTO", "XSBENCH_SCHEDULE_ASKED": "SCHEDULE_ASKED",
                "XSBENCH_SCHEDULE_AUTO": "SCHEDULE_AUTO",
-               "XSB_NOALLOC_MAX_LOOKUPS": "NOALLOC_MAX_LOOKUPS"}
+               "XSB_NOALLOC_MAX_LOOKUPS": "NOALLOC_MAX_LOOKUPS", "XSB_LINE_REGIONS": "N_LINE_REGIONS"}
 
 
 def test_abi_structs_and_constants_match_the_python_mirror(tmp_path):
@@ -431,3 +431,23 @@ def test_host_search_and_pick_mat_helpers(orc):
                 want_n = min(max(ub - 1, lo), hi - 1) if hi - lo > 1 else lo
                 got = lib.grid_search_nuclide(n, q, C.addressof(pts), lo, hi)
                 assert got == want_n == orc.grid_search_nuclide(n, q, pts, lo, hi)
+
+
+def test_table_sizes_fit_one_gpu():
+    """the resident tables of an upload, from xsb_hip_table_bytes (host
+    arithmetic, the sizes the upload records): no whole unionized index grid
+    is kept, so the XL problem on the unionized grid (355 x 238,847 points,
+    int32 per-material index tables) fits one 288 GB MI355X with FP32 and
+    BF16 tables too; the large problem's per-material tables are 16-bit"""
+    xl = X.table_bytes(X.make_inputs("XL", "unionized", "event"), build_fp32=True, build_bf16=True)
+    total = sum(v for k, v in xl.items() if k.startswith("bytes_"))
+    assert xl["bytes_index"] == 0
+    assert total < 250e9, total  # leaves room for the staging and sort buffers
+    n = 355 * 238847
+    # int32 entries (n_gp > 65536): rows of 324, 8, 4, 4, 28, 24 x 5, 12, 12 entries
+    assert xl["bytes_midx"] == n * 4 * (324 + 8 + 4 + 4 + 28 + 24 * 5 + 12 + 12)
+    lg = X.table_bytes(X.make_inputs("large", "unionized", "event"))
+    # uint16 entries: rows of 328, 8, 8, 8, 32, 24 x 5, 16, 16 entries
+    assert lg["bytes_index"] == 0 and lg["bytes_midx"] == 355 * 11303 * 2 * (328 + 8 * 3 + 32 + 24 * 5 + 16 * 2)
+    hs = X.table_bytes(X.make_inputs("large", "hash", "event"))
+    assert hs["bytes_index"] == 10000 * 356 * 4 and hs["bytes_midx"] == 0

@@ -153,6 +153,22 @@ class XsbResult(C.Structure):
                 "material_counts": [int(x) for x in self.material_counts]}
 
 
+# tables of xsb_lines.region_bytes, in order (XSB_LINE_REGIONS)
+LINE_REGIONS = ("ueg", "acc", "midx", "ivl", "ivl32", "index", "grid", "xs32", "xs16")
+N_LINE_REGIONS = len(LINE_REGIONS)
+
+
+class XsbLines(C.Structure):
+    _fields_ = [("bytes", C.c_int64), ("bytes_tables", C.c_int64), ("bytes_schedule", C.c_int64),
+                ("bytes_small", C.c_int64), ("launches", C.c_int64),
+                ("region_bytes", C.c_int64 * len(LINE_REGIONS))]
+
+    def as_dict(self):
+        d = {f: int(getattr(self, f)) for f, _ in self._fields_[:5]}
+        d["region_bytes"] = [int(x) for x in self.region_bytes]
+        return d
+
+
 class XsbenchOptions(C.Structure):
     """xsbench_options (include/xsbench.h): options of the host surface"""
     _fields_ = [
@@ -179,7 +195,9 @@ class XsbInfo(C.Structure):
 HIP_SYMBOLS = ("xsb_hip_upload", "xsb_hip_release", "xsb_hip_event", "xsb_hip_history",
                "xsb_hip_lookup", "xsb_hip_batched", "xsb_hip_balg", "xsb_hip_copy_probe",
                "xsb_hip_info", "xsb_hip_strerror", "xsb_hip_last_error", "xsb_hip_download",
-               "xsb_hip_macro_host", "xsb_hip_micro_host", "xsb_hip_event_noalloc")
+               "xsb_hip_macro_host", "xsb_hip_micro_host", "xsb_hip_event_noalloc",
+               "xsb_hip_lines_event", "xsb_hip_lines_history", "xsb_hip_lines_batched",
+               "xsb_hip_table_bytes")
 HOST_SYMBOLS = ("read_CLI", "print_inputs", "print_results", "grid_init_do_not_profile",
                 "load_num_nucs", "load_mats", "load_concs", "run_event_based_simulation",
                 "run_history_based_simulation", "run_event_based_simulation_optimization_1",
@@ -224,11 +242,18 @@ def hip_lib():
         _bind(L, "xsb_hip_history", C.c_int, [vp, C.POINTER(XsbHistoryArgs), C.POINTER(XsbResult)])
         _bind(L, "xsb_hip_event_noalloc", C.c_int,
               [vp, C.POINTER(XsbEventArgs), C.POINTER(XsbResult)])
+        _bind(L, "xsb_hip_lines_event", C.c_int, [vp, C.POINTER(XsbEventArgs), C.POINTER(XsbLines)])
+        _bind(L, "xsb_hip_lines_history", C.c_int,
+              [vp, C.POINTER(XsbHistoryArgs), C.POINTER(XsbLines)])
+        _bind(L, "xsb_hip_lines_batched", C.c_int,
+              [vp, C.POINTER(XsbBatchedArgs), C.POINTER(XsbLines)])
         _bind(L, "xsb_hip_lookup", C.c_int, [vp, C.c_int64, vp, vp, C.c_int32, vp, vp, vp, vp])
         _bind(L, "xsb_hip_batched", C.c_int, [vp, C.POINTER(XsbBatchedArgs), C.POINTER(XsbResult)])
         _bind(L, "xsb_hip_balg", C.c_int, [vp, C.POINTER(XsbEventArgs), C.POINTER(C.c_double)])
         _bind(L, "xsb_hip_copy_probe", C.c_int, [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double)])
         _bind(L, "xsb_hip_info", C.c_int, [vp, C.POINTER(XsbInfo)])
+        _bind(L, "xsb_hip_table_bytes", C.c_int, [C.POINTER(XsbTables), C.c_uint32,
+                                                 C.POINTER(XsbInfo)])
         _bind(L, "xsb_hip_download", C.c_int, [vp, vp, vp, vp])
         _bind(L, "xsb_hip_macro_host", C.c_int, [vp, C.c_int64, vp, vp, C.c_int32, vp])
         _bind(L, "xsb_hip_micro_host", C.c_int, [vp, C.c_int64, vp, vp, vp, vp, vp])
@@ -374,6 +399,17 @@ def tables_from_sd(inp, sd):
     t.nuclide_grid = C.cast(sd.nuclide_grid, C.c_void_p)
     t.length_nuclide_grid = sd.length_nuclide_grid
     return t
+
+
+def table_bytes(inp, build_fp32=False, build_bf16=False, idx32=False):
+    """the resident table sizes an upload of inp's problem would take
+    (xsb_hip_table_bytes: host arithmetic, no device)"""
+    t = material_tables(inp)
+    flags = (BUILD_FP32 if build_fp32 else 0) | (BUILD_BF16 if build_bf16 else 0)
+    flags |= BUILD_IDX32 if idx32 else 0
+    i = XsbInfo()
+    _check(hip_lib().xsb_hip_table_bytes(C.byref(t), flags, C.byref(i)))
+    return {f: getattr(i, f) for f, _ in XsbInfo._fields_}
 
 
 def material_tables(inp):
@@ -549,6 +585,46 @@ class Device:
         b = C.c_double()
         _check(self.lib.xsb_hip_balg(self.handle, C.byref(a), C.byref(b)))
         return b.value
+
+    def lines_event(self, lookups, first=0, kernel_id=K_MAT_E_SORT, precision=XS_FP64,
+                    seed=STARTING_SEED):
+        """compulsory bytes of an event run (xsb_hip_lines_event)"""
+        a = XsbEventArgs()
+        a.lookups = lookups
+        a.first_lookup = first
+        a.seed = seed
+        a.kernel_id = kernel_id
+        a.xs_precision = precision
+        o = XsbLines()
+        _check(self.lib.xsb_hip_lines_event(self.handle, C.byref(a), C.byref(o)))
+        return o.as_dict()
+
+    def lines_history(self, particles, lookups=34, precision=XS_FP64, seed=STARTING_SEED):
+        """compulsory bytes of a history run, over its rounds (xsb_hip_lines_history)"""
+        a = XsbHistoryArgs()
+        a.particles = particles
+        a.lookups_per_particle = lookups
+        a.kernel_id = K_MAT_E_SORT
+        a.xs_precision = precision
+        a.seed = seed
+        o = XsbLines()
+        _check(self.lib.xsb_hip_lines_history(self.handle, C.byref(a), C.byref(o)))
+        return o.as_dict()
+
+    def lines_batched(self, sizes, seeds, thresholds, precision=XS_FP64):
+        """compulsory bytes of a batched run (xsb_hip_lines_batched)"""
+        sizes = np.ascontiguousarray(sizes, np.int64)
+        seeds = np.ascontiguousarray(seeds, np.uint64)
+        thr = np.ascontiguousarray(thresholds, np.float64)
+        a = XsbBatchedArgs()
+        a.n_batches = len(sizes)
+        a.xs_precision = precision
+        a.sizes = sizes.ctypes.data
+        a.seeds = seeds.ctypes.data
+        a.thresholds = thr.ctypes.data
+        o = XsbLines()
+        _check(self.lib.xsb_hip_lines_batched(self.handle, C.byref(a), C.byref(o)))
+        return o.as_dict()
 
     def copy_probe(self, nbytes, reps=5):
         ms = C.c_double()

@@ -162,6 +162,21 @@ __global__ void k_sorted_check(const double *__restrict__ a, int64_t n, unsigned
 		atomicAdd(bad, (unsigned long long)b);
 }
 
+// XS values that are negative or NaN: with none (and no negative
+// concentration) every mixed-precision term of an interpolating lookup is
+// non-negative, which gives its channels relative error bounds.
+__global__ void k_negative_xs(const double *__restrict__ grid, int64_t n, unsigned long long *bad)
+{
+	uint32_t b = 0;
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+	     i += (int64_t)gridDim.x * blockDim.x)
+		for (int c = 1; c < kPointDoubles; c++)
+			b += (grid[i * kPointDoubles + c] >= 0.0) ? 0u : 1u;
+	b = wave_sum_u32(b);
+	if ((threadIdx.x & 63) == 0 && b)
+		atomicAdd(bad, (unsigned long long)b);
+}
+
 // Descents (or NaN) inside the nuclides' energy rows (points [nuc][k], stride
 // kPointDoubles): with none, a bracket search over a short range may count
 // the energies <= p instead of bisecting (the same index on a sorted row).
@@ -374,12 +389,15 @@ __global__ void k_repitch(const int32_t *__restrict__ src, int64_t rows, int32_t
 	}
 }
 
+// Index-grid rows staged at a time (host copies, device sweeps).
+constexpr int64_t kIndexChunkBytes = (int64_t)64 << 20;
+
 // Copies a rows x n int grid into the device's pitched layout through a
 // bounded staging buffer (contiguous host-to-device copies, re-pitched on
 // the device).
 static int upload_pitched(Device *d, const int32_t *host, int64_t rows, int64_t n, int64_t pitch)
 {
-	const int64_t stage_rows = std::max<int64_t>(1, ((int64_t)64 << 20) / (n * 4));
+	const int64_t stage_rows = std::max<int64_t>(1, kIndexChunkBytes / (n * 4));
 	void *stage = nullptr;
 	int rc = scratch_reserve(d, 0, (size_t)std::min(rows, stage_rows) * n * 4, &stage);
 	if (rc)
@@ -415,18 +433,25 @@ static int download(Device *d, double *grid, double *ueg, int32_t *index)
 		XSB_HIP(hipMemcpy(grid, d->d_grid, (size_t)n * kPointDoubles * 8, hipMemcpyDeviceToHost));
 	if (ueg && d->d_ueg)
 		XSB_HIP(hipMemcpy(ueg, d->d_ueg, (size_t)T.n_union * 8, hipMemcpyDeviceToHost));
-	if (index && d->d_index) {
+	// the hash grid's index grid is resident; the unionized one is swept again
+	// from the device's nuclide grid and unionized energies (GridInit.c:92-122)
+	if (index && (d->d_index || T.grid_type == kUnionized)) {
 		const int64_t rows = (T.grid_type == kUnionized) ? T.n_union : T.hash_bins;
-		const int64_t stage_rows = std::max<int64_t>(1, ((int64_t)64 << 20) / ((int64_t)T.n_iso * 4));
+		const int64_t stage_rows = std::max<int64_t>(1, kIndexChunkBytes / ((int64_t)T.n_iso * 4));
 		void *stage = nullptr;
 		int rc = scratch_reserve(d, 0, (size_t)std::min(rows, stage_rows) * T.n_iso * 4, &stage);
 		if (rc)
 			return rc;
 		for (int64_t r0 = 0; r0 < rows; r0 += stage_rows) {
 			const int64_t nr = std::min(stage_rows, rows - r0);
-			hipLaunchKernelGGL(k_unpitch, dim3(2048), dim3(256), 0, d->stream,
-			                   d->d_index + r0 * T.idx_pitch, nr, T.n_iso, T.idx_pitch,
-			                   (int32_t *)stage);
+			if (T.grid_type == kUnionized) {
+				if ((rc = sweep_index_rows(d, r0, nr, T.n_iso, (int32_t *)stage, d->stream)))
+					return rc;
+			} else {
+				hipLaunchKernelGGL(k_unpitch, dim3(2048), dim3(256), 0, d->stream,
+				                   d->d_index + r0 * T.idx_pitch, nr, T.n_iso, T.idx_pitch,
+				                   (int32_t *)stage);
+			}
 			XSB_HIP(hipGetLastError());
 			XSB_HIP(hipStreamSynchronize(d->stream));
 			XSB_HIP(hipMemcpy(index + r0 * T.n_iso, stage, (size_t)nr * T.n_iso * 4,
@@ -479,10 +504,10 @@ __global__ void k_interval_records(const double *__restrict__ grid, int64_t tota
 }
 
 // FP32 interval records (Tables::ivl32): {E_{k+1}, 1 / (E_{k+1} - E_k),
-// float(xs_{k+1})[5], float(xs_{k+1}) - float(xs_k) [5], pad}; zero for the last
-// point of each nuclide.  A reciprocal that is not a normal number (equal
-// energies, or a width above 2^1022) is stored as NaN: the lookup's error bound
-// is then NaN and the lookup is evaluated on the FP64 path.
+// float(xs_{k+1})[5], float(xs_k)[5], pad}; zero for the last point of each
+// nuclide.  A reciprocal that is not a normal number (equal energies, or a
+// width above 2^1022) is stored as NaN: the lookup's error bound is then NaN
+// and the lookup is evaluated on the FP64 path.
 __global__ void k_interval_records32(const double *__restrict__ grid, int64_t total, int32_t n_gp,
                                      double *__restrict__ ivl)
 {
@@ -498,10 +523,8 @@ __global__ void k_interval_records32(const double *__restrict__ grid, int64_t to
 		r[1] = last ? 0.0 : (__builtin_isnormal(rw) ? rw : __builtin_nan(""));
 #pragma unroll
 		for (int c = 0; c < 5; c++) {
-			const float xh = last ? 0.f : (float)lo[kPointDoubles + 1 + c];
-			const float xl = (float)lo[1 + c];
-			f[c] = xh;
-			f[5 + c] = last ? 0.f : xh - xl;
+			f[c] = last ? 0.f : (float)lo[kPointDoubles + 1 + c];
+			f[5 + c] = last ? 0.f : (float)lo[1 + c];
 		}
 		f[10] = 0.f;
 		f[11] = 0.f;
@@ -579,11 +602,14 @@ static int build_interval_records(Device *d)
 // Per-material copies of the unionized index grid (Tables::midx): each
 // lookup then reads its material's brackets from one contiguous row.
 // Brackets are at most n_gp - 1, so they are stored as uint16 when
-// n_gp <= 65536 unless `wide` asks for int32.
-static int build_material_index(Device *d, const xsb_tables *t, int64_t rows, int64_t pitch,
-                                bool wide)
+// n_gp <= 65536 unless `wide` asks for int32.  The full grid (n_union rows of
+// n_iso ints, 5.7 GB for the large problem) is never held on the device: rows
+// come a chunk at a time, copied from the host's index grid (and range-checked)
+// or, for grids built on the device, swept from its nuclide grid.
+static int build_material_index(Device *d, const xsb_tables *t, bool from_host, bool wide)
 {
 	Tables &T = d->T;
+	const int64_t rows = T.n_union, n_iso = T.n_iso;
 	const bool narrow = !wide && T.n_gp <= 65536;
 	T.midx_shift = narrow ? 1 : 2;
 	const int align = narrow ? 8 : 4;  // entries per 16 bytes
@@ -600,21 +626,51 @@ static int build_material_index(Device *d, const xsb_tables *t, int64_t rows, in
 		return fail(XSB_ERR_OOM, "hipMalloc of %zu bytes (material index tables) failed", bytes);
 	}
 	d->bytes_midx = (int64_t)bytes;
-	for (int m = 0; m < kMaterials; m++) {
-		if (!T.midx_pitch[m])
-			continue;
-		const int32_t *ml = d->d_mats + (int64_t)m * t->max_num_nucs;
-		if (narrow)
-			hipLaunchKernelGGL(k_material_index<uint16_t>, dim3(4096), dim3(256), 0, d->stream,
-			                   d->d_index, rows, (int32_t)pitch, ml, t->num_nucs[m],
-			                   T.midx_pitch[m], T.n_gp, (uint16_t *)d->d_midx + T.midx_off[m]);
-		else
-			hipLaunchKernelGGL(k_material_index<int32_t>, dim3(4096), dim3(256), 0, d->stream,
-			                   d->d_index, rows, (int32_t)pitch, ml, t->num_nucs[m],
-			                   T.midx_pitch[m], T.n_gp, (int32_t *)d->d_midx + T.midx_off[m]);
-		XSB_HIP(hipGetLastError());
+	const int64_t chunk = std::max<int64_t>(1, kIndexChunkBytes / (n_iso * 4));
+	void *stage = nullptr;
+	int rc = scratch_reserve(d, 0, (size_t)std::min(rows, chunk) * n_iso * 4, &stage);
+	if (rc)
+		return rc;
+	int32_t *rows_buf = (int32_t *)stage;
+	hipStream_t s = d->stream;
+	unsigned long long *bad = d->d_acc + 60;
+	XSB_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), s));
+	for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+		const int64_t nr = std::min(chunk, rows - r0);
+		if (from_host) {
+			// the staging buffer is reused by the next chunk: the copy waits
+			// for the previous chunk's kernels (same stream, synchronous copy)
+			XSB_HIP(hipStreamSynchronize(s));
+			XSB_HIP(hipMemcpy(rows_buf, t->index_grid + r0 * n_iso, (size_t)nr * n_iso * 4,
+			                  hipMemcpyHostToDevice));
+			hipLaunchKernelGGL(k_index_range, dim3(2048), dim3(256), 0, s, rows_buf, nr * n_iso,
+			                   (int32_t)(T.n_gp - 1), bad);
+			XSB_HIP(hipGetLastError());
+		} else if ((rc = sweep_index_rows(d, r0, nr, (int32_t)n_iso, rows_buf, s))) {
+			return rc;
+		}
+		for (int m = 0; m < kMaterials; m++) {
+			if (!T.midx_pitch[m])
+				continue;
+			const int32_t *ml = d->d_mats + (int64_t)m * t->max_num_nucs;
+			const int64_t off = T.midx_off[m] + r0 * T.midx_pitch[m];
+			if (narrow)
+				hipLaunchKernelGGL(k_material_index<uint16_t>, dim3(2048), dim3(256), 0, s,
+				                   rows_buf, nr, (int32_t)n_iso, ml, t->num_nucs[m],
+				                   T.midx_pitch[m], T.n_gp, (uint16_t *)d->d_midx + off);
+			else
+				hipLaunchKernelGGL(k_material_index<int32_t>, dim3(2048), dim3(256), 0, s,
+				                   rows_buf, nr, (int32_t)n_iso, ml, t->num_nucs[m],
+				                   T.midx_pitch[m], T.n_gp, (int32_t *)d->d_midx + off);
+			XSB_HIP(hipGetLastError());
+		}
 	}
-	XSB_HIP(hipStreamSynchronize(d->stream));
+	unsigned long long nbad = 0;
+	XSB_HIP(hipMemcpyAsync(&nbad, bad, sizeof(nbad), hipMemcpyDeviceToHost, s));
+	XSB_HIP(hipStreamSynchronize(s));
+	if (nbad)
+		return fail(XSB_ERR_INVALID, "index grid holds %llu entries outside [0, %d]", nbad,
+		            T.n_gp - 1);
 	T.midx = d->d_midx;
 	return XSB_OK;
 }
@@ -655,6 +711,31 @@ static int noalloc_setup(int device)
 		g_noalloc_rc[device] = XSB_OK + 1;
 	}
 	return g_noalloc_rc[device] - 1;
+}
+
+// Bytes of each resident table an upload of t with `flags` allocates (the
+// same sizes the upload records in xsb_info), computed on the host.
+static void table_bytes(const xsb_tables *t, uint32_t flags, xsb_info *o)
+{
+	memset(o, 0, sizeof(*o));
+	const int64_t n_iso = t->n_isotopes, n_gp = t->n_gridpoints, n = n_iso * n_gp;
+	const bool uni = t->grid_type == XSB_UNIONIZED;
+	o->grid_type = t->grid_type;
+	o->n_isotopes = n_iso;
+	o->n_gridpoints = n_gp;
+	o->n_union = uni ? n : 0;
+	o->bytes_fp64 = n * kPointDoubles * 8;
+	o->bytes_fp32 = (flags & XSB_BUILD_FP32) ? n * kRec32Doubles * 8 : 0;
+	o->bytes_bf16 = (flags & XSB_BUILD_BF16) ? n * kRec16Doubles * 8 : 0;
+	o->bytes_union = uni ? n * 8 : 0;
+	o->bytes_index = (t->grid_type == XSB_HASH) ? (int64_t)t->hash_bins * ((n_iso + 3) & ~3) * 4 : 0;
+	if (uni) {
+		const bool narrow = !(flags & XSB_BUILD_IDX32) && n_gp <= 65536;
+		const int align = narrow ? 8 : 4, width = narrow ? 2 : 4;
+		for (int m = 0; m < kMaterials; m++)
+			o->bytes_midx += n * (int64_t)((t->num_nucs[m] + align - 1) & ~(align - 1)) * width;
+		o->bytes_ivl = n * kIvlDoubles * 8 + ((flags & XSB_BUILD_FP32) ? n * kIvl32Doubles * 8 : 0);
+	}
 }
 
 static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
@@ -715,7 +796,7 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 			if ((rc = upload_array(&d->d_ueg, t->unionized_energy_array, n_union, &d->bytes_union)))
 				return rc;
 		}
-		if (t->grid_type != XSB_NUCLIDE) {
+		if (t->grid_type == XSB_HASH) {
 			const size_t bytes = (size_t)index_rows * pitch * sizeof(int32_t);
 			if (hipMalloc((void **)&d->d_index, bytes) != hipSuccess) {
 				(void)hipGetLastError();
@@ -759,7 +840,10 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	hipLaunchKernelGGL(k_rows_sorted_check, dim3(2048), dim3(256), 0, d->stream, d->d_grid,
 	                   n_iso, n_gp, d->d_acc + 62);
 	XSB_HIP(hipGetLastError());
-	unsigned long long chk[3] = {0, 0, 0};
+	hipLaunchKernelGGL(k_negative_xs, dim3(2048), dim3(256), 0, d->stream, d->d_grid, n_iso * n_gp,
+	                   d->d_acc + 63);
+	XSB_HIP(hipGetLastError());
+	unsigned long long chk[4] = {0, 0, 0, 0};
 	XSB_HIP(hipMemcpyAsync(chk, d->d_acc + 60, sizeof(chk), hipMemcpyDeviceToHost, d->stream));
 	XSB_HIP(hipStreamSynchronize(d->stream));
 	bad = chk[0];
@@ -779,13 +863,18 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	T.n_union = n_union;
 	T.n_iso = (int32_t)n_iso;
 	T.idx_pitch = (int32_t)pitch;
-	T.variant = kVariantDefault | (chk[2] == 0 ? kVariantSortedRows : 0);
+	bool conc_nonneg = true;
+	for (int m = 0; m < kMaterials; m++)
+		for (int j = 0; j < t->num_nucs[m]; j++)
+			conc_nonneg = conc_nonneg && t->concs[m * t->max_num_nucs + j] >= 0.0;
+	T.variant = kVariantDefault | (chk[2] == 0 ? kVariantSortedRows : 0) |
+	            (chk[3] == 0 && conc_nonneg ? kVariantNonNeg : 0);
 	T.n_gp = (int32_t)n_gp;
 	T.hash_bins = t->hash_bins;
 	T.max_num_nucs = t->max_num_nucs;
 	T.grid_type = t->grid_type;
 	if (t->grid_type == XSB_UNIONIZED &&
-	    (rc = build_material_index(d, t, n_union, pitch, (flags & XSB_BUILD_IDX32) != 0)))
+	    (rc = build_material_index(d, t, !gen, (flags & XSB_BUILD_IDX32) != 0)))
 		return rc;
 	T.hash_du = (t->hash_bins > 0) ? 1.0 / t->hash_bins : 0.0;
 	pick_thresholds(T.thr);
@@ -1224,6 +1313,18 @@ int xsb_hip_copy_probe(void *handle, int64_t bytes, int32_t reps, double *ms_per
 	XSB_HIP(hipEventRecord(d->ev[5], s));
 	XSB_HIP(hipStreamSynchronize(s));
 	*ms_per_copy = elapsed(d->ev[4], d->ev[5]) / reps;
+	return XSB_OK;
+}
+
+int xsb_hip_table_bytes(const xsb_tables *t, uint32_t flags, xsb_info *info)
+{
+	if (!info)
+		return fail(XSB_ERR_INVALID, "info is NULL");
+	int rc = validate_tables(t, flags | XSB_BUILD_GRIDS);  // sizes and materials only
+	if (rc)
+		return rc;
+	table_bytes(t, flags, info);
+	info->device = -1;
 	return XSB_OK;
 }
 

@@ -21,6 +21,7 @@ enum GridType : int { kUnionized = 0, kNuclide = 1, kHash = 2 };
 // Lookup-path choices (all give identical results).
 constexpr int kVariantWave = 1;  // wave-uniform path for material-uniform waves (unionized FP64)
 constexpr int kVariantSortedRows = 2;  // every nuclide's energy row is non-decreasing
+constexpr int kVariantNonNeg = 4;  // no negative XS value or concentration in the tables
 constexpr int kVariantDefault = kVariantWave;
 
 // Device view of the tables.  Passed by value as a kernel argument.

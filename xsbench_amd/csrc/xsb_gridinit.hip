@@ -17,7 +17,8 @@
 //            the run of equal energies before it, which is
 //            min(#{k >= 1 : E_k <= U[e]}, n_gp - 2) (every nuclide energy is
 //            itself in U, so the sweep has caught up by the end of each run),
-//            then follows the reference's per-row rule
+//            then follows the reference's per-row rule; swept a few hundred
+//            MB of rows at a time into the per-material tables, never whole
 //   hash     index_grid[b][i] = grid_search_nuclide(b * du) (GridInit.c:128-151)
 #include <vector>
 
@@ -75,8 +76,11 @@ __global__ void k_keys_to_energy(int64_t n, const uint64_t *__restrict__ key, do
 		ueg[j] = __longlong_as_double((long long)key[j]);
 }
 
-// Index grid rows [c R, (c+1) R) for nuclides blockIdx.y * 256 + threadIdx.x.
-__global__ __launch_bounds__(256) void k_index_sweep(const double *__restrict__ U, int64_t n_union,
+// Index grid rows [row0 + c R, row0 + (c+1) R) within [row0, row1), for
+// nuclides blockIdx.y * 256 + threadIdx.x; row e is written at
+// index + (e - row0) * pitch.
+__global__ __launch_bounds__(256) void k_index_sweep(const double *__restrict__ U, int64_t row0,
+                                                     int64_t row1,
                                                      const double *__restrict__ grid, int32_t n_iso,
                                                      int32_t n_gp, int64_t R, int32_t pitch,
                                                      int32_t *__restrict__ index)
@@ -84,8 +88,8 @@ __global__ __launch_bounds__(256) void k_index_sweep(const double *__restrict__ 
 	const int i = blockIdx.y * 256 + threadIdx.x;
 	if (i >= n_iso)
 		return;
-	const int64_t e0 = (int64_t)blockIdx.x * R;
-	const int64_t e1 = min(e0 + R, n_union);
+	const int64_t e0 = row0 + (int64_t)blockIdx.x * R;
+	const int64_t e1 = min(e0 + R, row1);
 	const double *E = grid + (int64_t)i * n_gp * kPointDoubles;  // energy k at E[6k]
 	int64_t r = e0;
 	while (r > 0 && U[r - 1] == U[r])
@@ -112,8 +116,23 @@ __global__ __launch_bounds__(256) void k_index_sweep(const double *__restrict__ 
 			e_high = E[(int64_t)(idx + 1) * kPointDoubles];
 		}
 		if (e >= e0)
-			index[e * pitch + i] = idx;
+			index[(e - row0) * pitch + i] = idx;
 	}
+}
+
+// Rows [row0, row0 + rows) of the unionized index grid (GridInit.c:92-122)
+// from the device's nuclide grid and unionized energies, into out with a
+// row pitch of `pitch` ints.
+int sweep_index_rows(Device *d, int64_t row0, int64_t rows, int32_t pitch, int32_t *out,
+                     hipStream_t s)
+{
+	const int64_t R = 2048;
+	const int32_t n_iso = d->T.n_iso;
+	const dim3 grid((unsigned)((rows + R - 1) / R), (unsigned)((n_iso + 255) / 256));
+	hipLaunchKernelGGL(k_index_sweep, grid, dim3(256), 0, s, d->d_ueg, row0, row0 + rows,
+	                   d->d_grid, n_iso, d->T.n_gp, R, pitch, out);
+	XSB_HIP(hipGetLastError());
+	return XSB_OK;
 }
 
 __global__ void k_hash_grid(const double *__restrict__ grid, int32_t n_iso, int32_t n_gp,
@@ -221,24 +240,19 @@ int build_grids_on_device(Device *d, const xsb_tables *t, int64_t pitch, std::ve
 	if (hipGetLastError() != hipSuccess)
 		return done(fail(XSB_ERR_HIP, "nuclide grid gather launch failed"));
 
-	if (t->grid_type != XSB_NUCLIDE) {
-		const int64_t rows = (t->grid_type == XSB_UNIONIZED) ? n : t->hash_bins;
+	// the unionized index grid is never held whole: the per-material tables
+	// are built from swept row chunks (sweep_index_rows) after this returns
+	if (t->grid_type == XSB_HASH) {
+		const int64_t rows = t->hash_bins;
 		const size_t bytes = (size_t)rows * pitch * sizeof(int32_t);
 		if ((rc = device_alloc((void **)&d->d_index, bytes, "index grid")))
 			return done(rc);
 		d->bytes_index = (int64_t)bytes;
 		if (hipMemsetAsync(d->d_index, 0, bytes, s) != hipSuccess)
 			return done(fail(XSB_ERR_HIP, "index grid clear failed"));
-		if (t->grid_type == XSB_UNIONIZED) {
-			const int64_t R = 2048;
-			const dim3 grid((unsigned)((n + R - 1) / R), (unsigned)((n_iso + 255) / 256));
-			hipLaunchKernelGGL(k_index_sweep, grid, dim3(256), 0, s, d->d_ueg, n, d->d_grid,
-			                   (int32_t)n_iso, (int32_t)n_gp, R, (int32_t)pitch, d->d_index);
-		} else {
-			hipLaunchKernelGGL(k_hash_grid, dim3(nb), dim3(256), 0, s, d->d_grid, (int32_t)n_iso,
-			                   (int32_t)n_gp, t->hash_bins, 1.0 / t->hash_bins, (int32_t)pitch,
-			                   d->d_index);
-		}
+		hipLaunchKernelGGL(k_hash_grid, dim3(nb), dim3(256), 0, s, d->d_grid, (int32_t)n_iso,
+		                   (int32_t)n_gp, t->hash_bins, 1.0 / t->hash_bins, (int32_t)pitch,
+		                   d->d_index);
 		if (hipGetLastError() != hipSuccess)
 			return done(fail(XSB_ERR_HIP, "index grid launch failed"));
 	}

@@ -119,9 +119,13 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 void finish_sorted_event(const Device *d, const xsb_event_args *a, xsb_result *r);
 
 // ---- grid construction on the device (xsb_gridinit.hip): allocates and
-// fills d_grid, d_ueg (unionized) and d_index (rows padded to `pitch`), and
-// returns each nuclide's max |xs| ----
+// fills d_grid, d_ueg (unionized) or d_index (hash grid, rows padded to
+// `pitch`), and returns each nuclide's max |xs| ----
 int build_grids_on_device(Device *d, const xsb_tables *t, int64_t pitch, std::vector<double> *xs_max);
+// Rows [row0, row0 + rows) of the unionized index grid, swept from the
+// device's nuclide grid and unionized energies, into out (row pitch `pitch`).
+int sweep_index_rows(Device *d, int64_t row0, int64_t rows, int32_t pitch, int32_t *out,
+                     hipStream_t s);
 
 // ---- history in material-energy sorted rounds (xsb_history.hip) ----
 int run_history_rounds(Device *d, const xsb_history_args *a, xsb_result *r);

@@ -1,5 +1,5 @@
 // xsb_mixed.h -- mixed-precision macroscopic XS (FP32 or BF16 XS tables,
-// FP64 energies, FP32 Kahan accumulation) with an FP64 fallback that keeps
+// FP64 energies, FP32 terms summed in FP64) with an FP64 fallback that keeps
 // every decision identical to the FP64 reference path.
 //
 // Table records (built on the device from the FP64 AoS at upload):
@@ -9,24 +9,40 @@
 // Energies stay FP64 so that brackets and the interpolation fraction are the
 // reference's exactly; only the 5 channel values are rounded.
 //
-// Error bound (per lookup, shared by the 5 channels).  With u the unit
-// roundoff of the table format (2^-24 FP32, 2^-8 BF16), u32 = 2^-24, M_j an
-// upper bound of |xs| for nuclide j and f_j the FP64 fraction, every term
-// conc_j * xs_j of the reference differs from ours by at most
-// conc_j M_j (1 + 2|f_j|) (u + 7 u32 + 8 u64) (5 u64 with the quotient as
-// fraction, 3 more with the interval records' reciprocal), and the Kahan sum adds at most
-// 2 u32 |S| + n u32^2 sum|t|; the reference's own ordered FP64 sum differs from
-// the exact sum by (n-1) u64 sum|r|.  So |S_c - R_c| <= B with
+// Each nuclide's term is conc * ((1 - f) xs_hi + f xs_lo), the reference's
+// hi - f (hi - lo) regrouped, with f = (E_hi - p) / (E_hi - E_lo) and 1 - f both
+// formed in FP64 and rounded once to FP32: the two weights carry the relative
+// error of one rounding each, so a term's error is relative to the term.
+//
+// Error bounds.  With u the unit roundoff of the table format (2^-24 FP32,
+// 2^-8 BF16), u32 = 2^-24, u64 = 2^-53, M_j an upper bound of |xs| for
+// nuclide j and f_j the fraction, every FP32 term of ours differs from the
+// reference's conc_j (hi - f_j (hi - lo)) by at most
+// conc_j (|hi| |1 - f_j| + |lo| |f_j|) (u + 5 u32 + 8 u64) + 5 u64 conc_j M_j (the
+// table values, the two weights, the concentration, the weights' products and
+// the term's two roundings; the weights' FP64 fraction within a few u64 of
+// the reference's), the FP64 sum of the terms adds (n-1) u64 sum|t|, and the
+// reference's own ordered FP64 sum differs from the exact sum by
+// (n-1) u64 sum|r|.  So, for every channel, |S_c - R_c| <= B with
 //   B = 1.01 * A * (u + 12 u32),   A = sum_j conc_j M_j (1 + 2|f_j|)
 // (the 1.01 absorbs the FP32 evaluation of A itself and the rounding of
-// conc_j M_j, for n < 10^5).  A lookup whose argmax gap is <= 2B, or (history
-// mode) with some |S_c - 1| <= B, is recomputed in FP64, and so is, with
-// FP32 tables, a lookup with some channel where B > kRelTol |S_c| (sums whose
-// terms nearly cancel, from extrapolation below a nuclide's first point):
-// every FP32 result is then within relative error kRelTol of the FP64
-// reference, channel by channel.  B is a worst case over the term signs and
-// rounding directions (about 8e-7 A); the errors actually seen are ~3e-8
-// relative on average.  A recomputed lookup returns the FP64 sums.
+// conc_j M_j, for n < 10^4).  When the tables hold no negative XS or
+// concentration (checked at upload) and every weight of the lookup lies in
+// [0, 1] (p inside each bracket: always on the unionized grid), each term is
+// non-negative and the first part sums to (u + 5 u32) sum_j T_j with T_j >= 0
+// the exact terms, i.e. channel by channel
+//   b_c = 1.01 * (u + 8 u32) |S_c| + kRefTol * A
+// (kRefTol covers the FP64 roundings, (2n + 13) u64 A, for n < 4 * 10^4;
+// u + 8 u32 covers BF16's double rounding FP64 -> FP32 -> BF16).
+// Otherwise b_c = B.  A lookup is recomputed in FP64 when its argmax is not
+// separated (S_am - S_k <= b_am + b_k for some k), in history mode when some
+// |S_c - 1| <= b_c, and, with FP32 tables, when some b_c > kRelTol |S_c|
+// (sums whose terms nearly cancel, from extrapolation outside a nuclide's
+// points): every FP32 result is then within relative error kRelTol of the
+// FP64 reference, channel by channel, and the hash is the reference's.  B is
+// a worst case over signs and rounding directions (about 8e-7 A for FP32);
+// the errors actually seen are ~3e-8 relative on average.  A recomputed
+// lookup returns the FP64 sums.
 #pragma once
 
 #include "xsb_device.h"
@@ -34,7 +50,8 @@
 namespace xsb {
 
 constexpr int kRec32Doubles = 4;  // 32 B record
-constexpr double kRelTol = 1e-5;  // FP32 tables: certified relative error per channel
+constexpr double kRelTol = 1e-6;  // FP32 tables: certified relative error per channel
+constexpr double kRefTol = 1e-11; // the FP64 roundings of both sums, relative to A
 constexpr int kRec16Doubles = 3;  // 24 B record
 constexpr int kIvl32Doubles = 8;  // 64 B FP32 interval record (Tables::ivl32)
 
@@ -80,53 +97,68 @@ __device__ __forceinline__ void load_xs(const double *rec, float x[5])
 	}
 }
 
-// The Kahan step for a corrected term y = t - comp already formed.
-__device__ __forceinline__ void kahan_add_corrected(float &sum, float &comp, float y)
+// The two FP32 weights of a nuclide's term from the FP64 fraction f:
+// fr = RN(f), gr = RN(1 - f); wmin keeps the smallest weight of the lookup
+// (every term is a non-negative combination when it stays >= 0).
+__device__ __forceinline__ void mixed_weights(double f64, float &fr, float &gr, float &wmin)
 {
-	const float s = sum + y;
-	comp = (s - sum) - y;
-	sum = s;
+	fr = (float)f64;
+	gr = (float)(1.0 - f64);
+	wmin = fminf(wmin, fminf(fr, gr));
 }
 
-// One nuclide's term: interpolate the 5 channels between (e_lo, xl) and
-// (e_hi, xh) with the FP64 fraction, add x * c to the Kahan sums, and c_bound
-// (1 + 2|f|) to the bound accumulator A.  The Kahan step's corrected term
-// y = x c - comp is formed as (xh c - comp) - (fr c) d, d = xh - xl, with two
-// fused operations: before the Kahan step's own rounding of y, the roundings
-// of fr c and of xh c - comp err by at most u32 c M (1 + 2|f|) + u32 |comp|,
-// where rounding x and x c separately may err by 2 u32 c M (1 + 2|f|); the
-// bound above counts the latter.
-__device__ __forceinline__ void mixed_term(double e_lo, const float xl[5], double e_hi,
-                                           const float xh[5], double p, float c, float cb,
-                                           float sum[5], float comp[5], float &A)
+// Adds one nuclide's FP32 term c ((1 - f) xh + f xl) to the FP64 sums and its
+// bound weight c_bound (1 + 2|f|) to A.
+__device__ __forceinline__ void mixed_add(const float xl[5], const float xh[5], float fr, float gr,
+                                          float c, float cb, double sum[5], float &A)
 {
-	const double fr64 = (e_hi - p) / (e_hi - e_lo);
-	const float fr = (float)fr64;
-	const float frc = fr * c;
+	const float fc = fr * c, gc = gr * c;
 #pragma unroll
 	for (int k = 0; k < 5; k++)
-		kahan_add_corrected(sum[k], comp[k],
-		                    __builtin_fmaf(-frc, xh[k] - xl[k], __builtin_fmaf(xh[k], c, -comp[k])));
+		sum[k] += (double)__builtin_fmaf(xl[k], fc, xh[k] * gc);
 	A += cb * (1.0f + 2.0f * fabsf(fr));
 }
 
-// One nuclide's term from an FP32 interval record R, added to the Kahan sums
-// with mixed_term's operations.  The FP32 differences were rounded once when
-// the record was built, exactly as mixed_term rounds them; the FP64 fraction
-// is the numerator times the record's rounded reciprocal width, within 3 u64
-// (relative) of the quotient mixed_term takes.
-__device__ __forceinline__ float ivl32_fraction(const double *__restrict__ R, double p)
+// The FP64 fraction (e_hi - p) / (e_hi - e_lo) through the hardware
+// reciprocal and two Newton steps: relative error far below 2^-40 (the
+// bounds above leave 2 u32 of slack), a few instructions instead of the
+// correctly rounded division; a zero width gives an infinite or NaN fraction,
+// and so an infinite or NaN bound: the lookup falls back to FP64.
+__device__ __forceinline__ double mixed_fraction(double e_lo, double e_hi, double p)
 {
-	return (float)((R[0] - p) * R[1]);
+	const double w = e_hi - e_lo;
+	double r = __builtin_amdgcn_rcp(w);
+	r = __builtin_fma(r, __builtin_fma(-w, r, 1.0), r);
+	r = __builtin_fma(r, __builtin_fma(-w, r, 1.0), r);
+	return (e_hi - p) * r;
 }
 
-__device__ __forceinline__ void ivl32_terms(const double *__restrict__ R, float frc, float c,
-                                            const float comp[5], float y[5])
+// One nuclide's term between the points (e_lo, xl) and (e_hi, xh).
+__device__ __forceinline__ void mixed_term(double e_lo, const float xl[5], double e_hi,
+                                           const float xh[5], double p, float c, float cb,
+                                           double sum[5], float &A, float &wmin)
 {
-	const float *f = reinterpret_cast<const float *>(R + 2);  // xh[5], d[5]
+	float fr, gr;
+	mixed_weights(mixed_fraction(e_lo, e_hi, p), fr, gr, wmin);
+	mixed_add(xl, xh, fr, gr, c, cb, sum, A);
+}
+
+// One nuclide's term from an FP32 interval record R = {E_hi, RN(1 / (E_hi -
+// E_lo)), xs_hi[5], xs_lo[5]}: the FP64 fraction is the numerator times the
+// rounded reciprocal, within 3 u64 (relative) of the quotient; the XS are the
+// points' own FP32 values.
+__device__ __forceinline__ double ivl32_fraction(const double *__restrict__ R, double p)
+{
+	return (R[0] - p) * R[1];
+}
+
+__device__ __forceinline__ void ivl32_terms(const double *__restrict__ R, float fc, float gc,
+                                            float t[5])
+{
+	const float *f = reinterpret_cast<const float *>(R + 2);  // xh[5], xl[5]
 #pragma unroll
 	for (int k = 0; k < 5; k++)
-		y[k] = __builtin_fmaf(-frc, f[5 + k], __builtin_fmaf(f[k], c, -comp[k]));
+		t[k] = __builtin_fmaf(f[5 + k], fc, f[k] * gc);
 }
 
 // mixed_term on an FP32 interval record: the record shared by the wave (R),
@@ -135,67 +167,75 @@ __device__ __forceinline__ void ivl32_terms(const double *__restrict__ R, float 
 // (scalar) records and each lane keeps its own, which is cheaper than first
 // selecting the records' 30 dwords per lane at three instructions each.
 __device__ __forceinline__ void mixed_term_ivl(const double *__restrict__ R, double p, float c,
-                                               float cb, float sum[5], float comp[5], float &A)
+                                               float cb, double sum[5], float &A, float &wmin)
 {
-	const float fr = ivl32_fraction(R, p);
-	float y[5];
-	ivl32_terms(R, fr * c, c, comp, y);
+	float fr, gr;
+	mixed_weights(ivl32_fraction(R, p), fr, gr, wmin);
+	float t[5];
+	ivl32_terms(R, fr * c, gr * c, t);
 #pragma unroll
 	for (int k = 0; k < 5; k++)
-		kahan_add_corrected(sum[k], comp[k], y[k]);
+		sum[k] += (double)t[k];
 	A += cb * (1.0f + 2.0f * fabsf(fr));
 }
 
 __device__ __forceinline__ void mixed_term_ivl2(const double *__restrict__ Q, bool sel, double p,
-                                                float c, float cb, float sum[5], float comp[5],
-                                                float &A)
+                                                float c, float cb, double sum[5], float &A,
+                                                float &wmin)
 {
-	const float fa = ivl32_fraction(Q, p);
-	const float fb = ivl32_fraction(Q + kIvl32Doubles, p);
-	float ya[5], yb[5];
-	ivl32_terms(Q, fa * c, c, comp, ya);
-	ivl32_terms(Q + kIvl32Doubles, fb * c, c, comp, yb);
+	const double fa64 = ivl32_fraction(Q, p), fb64 = ivl32_fraction(Q + kIvl32Doubles, p);
+	const float fa = (float)fa64, ga = (float)(1.0 - fa64);
+	const float fb = (float)fb64, gb = (float)(1.0 - fb64);
+	float ta[5], tb[5];
+	ivl32_terms(Q, fa * c, ga * c, ta);
+	ivl32_terms(Q + kIvl32Doubles, fb * c, gb * c, tb);
 #pragma unroll
 	for (int k = 0; k < 5; k++)
-		kahan_add_corrected(sum[k], comp[k], sel ? yb[k] : ya[k]);
-	A += cb * (1.0f + 2.0f * fabsf(sel ? fb : fa));
+		sum[k] += (double)(sel ? tb[k] : ta[k]);
+	const float fr = sel ? fb : fa, gr = sel ? gb : ga;
+	A += cb * (1.0f + 2.0f * fabsf(fr));
+	wmin = fminf(wmin, fminf(fr, gr));
 }
 
-// The decision on the mixed sums (see the bound above): returns the argmax
+// The decision on the mixed sums (see the bounds above): returns the argmax
 // the FP64 reference would return; macro[] receives the mixed-precision sums,
-// or the FP64 sums when the lookup fell back (*fell_back = 1).  fp64(m)
-// writes this lane's FP64 sums.  WAVE: fp64 is a wave-cooperative path that
-// every lane of the wave must enter, so it runs for the whole wave as soon as
-// one lane needs it (cheaper than one lane's FP64 loop with its wave idle).
+// or the FP64 sums when the lookup fell back (*fell_back = 1); *bound the
+// largest channel bound.  nonneg: the tables hold no negative XS or
+// concentration.  fp64(m) writes this lane's FP64 sums.  WAVE: fp64 is a
+// wave-cooperative path that every lane of the wave must enter, so it runs
+// for the whole wave as soon as one lane needs it (cheaper than one lane's
+// FP64 loop with its wave idle).
 template <int PREC, bool HISTORY, bool WAVE, class F64>
-__device__ __forceinline__ int mixed_finish(const float sum[5], float A, double macro[5],
-                                            uint32_t *fell_back, double *bound, F64 &&fp64)
+__device__ __forceinline__ int mixed_finish(const double s[5], float A, float wmin, bool nonneg,
+                                            double macro[5], uint32_t *fell_back, double *bound,
+                                            F64 &&fp64)
 {
-	constexpr float kU = (PREC == XSB_XS_FP32) ? 0x1p-24f : 0x1p-8f;
+	constexpr double kU = (PREC == XSB_XS_FP32) ? 0x1p-24 : 0x1p-8;
 	// argmax of the mixed sums (max = -1.0, strict '>', as the reference)
-	double s[5];
-#pragma unroll
-	for (int k = 0; k < 5; k++)
-		s[k] = (double)sum[k];
 	int am = argmax5(s);
-	const double B = 1.01 * (double)A * ((double)kU + 12.0 * 0x1p-24);
+	const double B = 1.01 * (double)A * (kU + 12.0 * 0x1p-24);
+	const bool rel = nonneg && wmin >= 0.f;
+	double b[5], bmax = 0.0;
+#pragma unroll
+	for (int k = 0; k < 5; k++) {
+		b[k] = rel ? 1.01 * (kU + 8.0 * 0x1p-24) * fabs(s[k]) + kRefTol * (double)A : B;
+		bmax = fmax(bmax, b[k]);
+	}
 	if (bound)
-		*bound = B;
+		*bound = rel ? bmax : B;
 	bool ambiguous = false;
 #pragma unroll
 	for (int k = 0; k < 5; k++) {
-		if (k != am && !(s[am] - s[k] > 2.0 * B))
+		if (k != am && !(s[am] - s[k] > b[am] + b[k]))
 			ambiguous = true;
-		if (HISTORY && fabs(s[k] - 1.0) <= B)
+		if (HISTORY && !(fabs(s[k] - 1.0) > b[k]))
 			ambiguous = true;
-		// B <= kRelTol |S_c| certifies |S_c - R_c| <= kRelTol |S_c| / (1 - kRelTol)
-		if (PREC == XSB_XS_FP32 && !(B <= kRelTol * (1.0 - kRelTol) * fabs(s[k])))
+		// b_c <= kRelTol (1 - kRelTol) |S_c| certifies |S_c - R_c| <= kRelTol |R_c|
+		if (PREC == XSB_XS_FP32 && !(b[k] <= kRelTol * (1.0 - kRelTol) * fabs(s[k])))
 			ambiguous = true;
 	}
-	// the reference's argmax needs max > -1.0 for a non-zero index; every
-	// sum is >= -B here only if the XS are non-negative, so treat a sum
-	// within B of -1.0 as ambiguous too
-	if (s[am] <= -1.0 + B)
+	// the reference's argmax needs max > -1.0 for a non-zero index
+	if (!(s[am] > -1.0 + b[am]))
 		ambiguous = true;
 #pragma unroll
 	for (int k = 0; k < 5; k++)
@@ -233,16 +273,15 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 	const float *__restrict__ cbnd = ceff + T.max_num_nucs * kMaterials;
 	const int32_t *__restrict__ ml = T.mats + mat * T.max_num_nucs;
 	const uint8_t *__restrict__ mrow = (GRID == kUnionized) ? ueg_row(T, mat, idx) : nullptr;
-	float sum[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-	float A = 0.f;
+	double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+	float A = 0.f, wmin = 1.f;
 	auto step = [&](int j) {
 		const int nuc = ml[j];
 		const double *row = mixed_row<PREC>(T, nuc);
 		const int32_t lower = nuclide_lower<GRID, REC, HISTORY>(T, row, nuc, mrow, j, idx, p);
 		if (PREC == XSB_XS_FP32 && GRID == kUnionized) {  // FP32 interval records
 			mixed_term_ivl(T.ivl32 + ((int64_t)nuc * T.n_gp + lower) * kIvl32Doubles, p, ceff[j],
-			               cbnd[j], sum, comp, A);
+			               cbnd[j], sum, A, wmin);
 			return;
 		}
 		const double *lo = row + (int64_t)lower * REC;
@@ -250,7 +289,7 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 		float xl[5], xh[5];
 		load_xs<PREC>(lo, xl);
 		load_xs<PREC>(hi, xh);
-		mixed_term(lo[0], xl, hi[0], xh, p, ceff[j], cbnd[j], sum, comp, A);
+		mixed_term(lo[0], xl, hi[0], xh, p, ceff[j], cbnd[j], sum, A, wmin);
 	};
 	if constexpr (HISTORY) {
 		// two nuclides per trip, as macro_xs_fp64 in the history kernels
@@ -261,9 +300,10 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 		for (int j = 0; j < nn; j++)
 			step(j);
 	}
-	return mixed_finish<PREC, HISTORY, false>(sum, A, macro, fell_back, bound, [&](double m64[5]) {
-		macro_xs_fp64<GRID, HISTORY>(T, p, mat, idx, m64);
-	});
+	return mixed_finish<PREC, HISTORY, false>(sum, A, wmin, (T.variant & kVariantNonNeg) != 0, macro,
+	                                          fell_back, bound, [&](double m64[5]) {
+		                                          macro_xs_fp64<GRID, HISTORY>(T, p, mat, idx, m64);
+	                                          });
 }
 
 // One nuclide of the wave-uniform path on the mixed tables, bracket k per
@@ -274,8 +314,8 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 // instead, as ueg_wave_nuclide does in FP64.
 template <int PREC, bool SHARE3 = true, bool IVL = false>
 __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc, int k, double p,
-                                                       float c, float cb, float sum[5],
-                                                       float comp[5], float &A)
+                                                       float c, float cb, double sum[5],
+                                                       float &A, float &wmin)
 {
 	constexpr int REC = (PREC == XSB_XS_FP32) ? kRec32Doubles : kRec16Doubles;
 	const int32_t n_gp = T.n_gp;
@@ -287,16 +327,16 @@ __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc,
 		const uint32_t du = (uint32_t)d;
 		const double *R = T.ivl32 + ((int64_t)nuc * n_gp + k0) * kIvl32Doubles;
 		if (__ballot(du != 0u) == 0ULL) {
-			mixed_term_ivl(R, p, c, cb, sum, comp, A);
+			mixed_term_ivl(R, p, c, cb, sum, A, wmin);
 		} else if (SHARE3 && (__ballot(du > 1u) == 0ULL || __ballot(du + 1u > 1u) == 0ULL)) {
 			// two records shared; each lane takes its own
 			const bool back = __ballot(du > 1u) != 0ULL;
 			mixed_term_ivl2(back ? R - kIvl32Doubles : R, (back ? du + 1u : du) != 0u, p, c, cb,
-			                sum, comp, A);
+			                sum, A, wmin);
 			__builtin_amdgcn_sched_barrier(0);
 		} else {
-			mixed_term_ivl(T.ivl32 + ((int64_t)nuc * n_gp + k) * kIvl32Doubles, p, c, cb, sum,
-			               comp, A);
+			mixed_term_ivl(T.ivl32 + ((int64_t)nuc * n_gp + k) * kIvl32Doubles, p, c, cb, sum, A,
+			               wmin);
 		}
 		return;
 	}
@@ -306,7 +346,7 @@ __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc,
 	if (__ballot(d != 0) == 0ULL) {
 		load_xs<PREC>(P, x0);
 		load_xs<PREC>(P + REC, x1);
-		mixed_term(P[0], x0, P[REC], x1, p, c, cb, sum, comp, A);
+		mixed_term(P[0], x0, P[REC], x1, p, c, cb, sum, A, wmin);
 	} else if (SHARE3 &&
 	           (__ballot(d != 0 && d != 1) == 0ULL || __ballot(d != 0 && d != -1) == 0ULL)) {
 		// three records Q, Q+1, Q+2 shared; each lane takes its pair
@@ -323,13 +363,13 @@ __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc,
 			xl[w] = sel ? x1[w] : x0[w];
 			xh[w] = sel ? x2[w] : x1[w];
 		}
-		mixed_term(sel ? e1 : e0, xl, sel ? e2 : e1, xh, p, c, cb, sum, comp, A);
+		mixed_term(sel ? e1 : e0, xl, sel ? e2 : e1, xh, p, c, cb, sum, A, wmin);
 		__builtin_amdgcn_sched_barrier(0);
 	} else {
 		const double *L = row + (int64_t)k * REC;
 		load_xs<PREC>(L, x0);
 		load_xs<PREC>(L + REC, x1);
-		mixed_term(L[0], x0, L[REC], x1, p, c, cb, sum, comp, A);
+		mixed_term(L[0], x0, L[REC], x1, p, c, cb, sum, A, wmin);
 	}
 }
 
@@ -342,15 +382,15 @@ __device__ __forceinline__ int macro_ueg_wave_mixed(const Tables &T, double p, i
 {
 	const float *__restrict__ ceff = mixed_consts<PREC>(T, m);
 	const float *__restrict__ cbnd = ceff + T.max_num_nucs * kMaterials;
-	float sum[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-	float A = 0.f;
+	double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+	float A = 0.f, wmin = 1.f;
 	ueg_wave_rows<CHUNK>(T, m, idx, [&](int j, int nuc, int k) {
-		ueg_wave_nuclide_mixed<PREC, SHARE3, true>(T, nuc, k, p, ceff[j], cbnd[j], sum, comp, A);
+		ueg_wave_nuclide_mixed<PREC, SHARE3, true>(T, nuc, k, p, ceff[j], cbnd[j], sum, A, wmin);
 	});
-	return mixed_finish<PREC, HISTORY, true>(sum, A, macro, fell_back, bound, [&](double m64[5]) {
-		macro_ueg_wave<CHUNK, SHARE3>(T, p, m, idx, m64);
-	});
+	return mixed_finish<PREC, HISTORY, true>(sum, A, wmin, (T.variant & kVariantNonNeg) != 0, macro,
+	                                         fell_back, bound, [&](double m64[5]) {
+		                                         macro_ueg_wave<CHUNK, SHARE3>(T, p, m, idx, m64);
+	                                         });
 }
 
 // calculate_macro_xs, hash grid, mixed tables, for a wave whose lanes share
@@ -363,15 +403,15 @@ __device__ __forceinline__ int macro_hash_wave_mixed(const Tables &T, double p, 
 	constexpr int REC = (PREC == XSB_XS_FP32) ? kRec32Doubles : kRec16Doubles;
 	const float *__restrict__ ceff = mixed_consts<PREC>(T, m);
 	const float *__restrict__ cbnd = ceff + T.max_num_nucs * kMaterials;
-	float sum[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-	float comp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-	float A = 0.f;
+	double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+	float A = 0.f, wmin = 1.f;
 	hash_wave_rows<REC>(T, mixed_row<PREC>(T, 0), p, m, idx, [&](int j, int nuc, int k) {
-		ueg_wave_nuclide_mixed<PREC, SHARE3>(T, nuc, k, p, ceff[j], cbnd[j], sum, comp, A);
+		ueg_wave_nuclide_mixed<PREC, SHARE3>(T, nuc, k, p, ceff[j], cbnd[j], sum, A, wmin);
 	});
-	return mixed_finish<PREC, HISTORY, true>(sum, A, macro, fell_back, bound, [&](double m64[5]) {
-		macro_hash_wave<SHARE3>(T, p, m, idx, m64);
-	});
+	return mixed_finish<PREC, HISTORY, true>(sum, A, wmin, (T.variant & kVariantNonNeg) != 0, macro,
+	                                         fell_back, bound, [&](double m64[5]) {
+		                                         macro_hash_wave<SHARE3>(T, p, m, idx, m64);
+	                                         });
 }
 
 }  // namespace xsb

@@ -1,11 +1,12 @@
 """Fixed workload for rocprofv3 runs of the lookup path.
 
-Runs one workload a few times plus a plain device copy of known size, so that
-per-dispatch hardware counters of the lookup kernel can be scaled by what the
-same counters read for the copy (which moves exactly 2 x COPY_BYTES), and
-writes what the summary needs (algorithmic bytes, sizes) as JSON.
+Runs one workload a few times plus a plain device copy of known size (a
+check of the byte counters: the copy reads and writes exactly COPY_BYTES),
+and writes what the summary needs as JSON: the lookup kernel's name and HIP
+event time, its compulsory bytes per launch (the distinct 128-byte lines its
+lookups read, xsb_hip_lines_*) and, for event runs, the algorithmic bytes.
 
-    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT -o run -- \\
+    rocprofv3 --pmc TCC_EA0_RDREQ_sum TCC_EA0_RDREQ_32B_sum --output-format csv -d OUT -o run -- \\
         python3 xsbench_amd/profile_driver.py --workload large_unionized --kernel 6 --meta OUT/meta.json
 """
 import argparse
@@ -66,10 +67,22 @@ def main():
           f"({2 * COPY_BYTES / copy_ms / 1e6:.0f} GB/s)", flush=True)
     if args.meta:
         meta = {"workload": args.workload, "kernel_id": args.kernel, "precision": args.precision,
-                "copy_bytes": COPY_BYTES, "lookup_ms": min(ms)}
+                "copy_bytes": COPY_BYTES, "lookup_ms": min(ms), "method": method}
         if method == "event":
+            meta["kernel"] = {0: "k_event_fused", 1: "k_lookup_list"}.get(args.kernel, "k_lookup_ids")
             meta["lookups"] = args.lookups
             meta["balg_bytes_per_lookup"] = dev.balg(args.lookups, precision=prec)
+            comp = dev.lines_event(args.lookups, kernel_id=args.kernel, precision=prec)
+        elif method == "batched":
+            meta["kernel"] = "k_bsorted"
+            meta["lookups"] = int(sizes.sum())
+            comp = dev.lines_batched(sizes, seeds, thr, precision=prec)
+        else:
+            meta["kernel"] = "k_hround"
+            meta["lookups"] = args.particles * 34
+            comp = dev.lines_history(args.particles, 34, precision=prec)
+        meta["compulsory_bytes_per_launch"] = comp["bytes"] / comp["launches"]
+        meta["compulsory"] = comp
         with open(args.meta, "w") as f:
             json.dump(meta, f)
     dev.close()
