@@ -23,7 +23,7 @@ sys.path.insert(0, ROOT)
 LOOKUPS = 17_000_000
 HASH_LARGE = 952131
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_lookup_large_unionized.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "roofline_large_unionized_k6_fp64.txt")
 
 
 def log(msg):
@@ -121,6 +121,8 @@ def batched_variants(X, dev):
     ref_vs = None
     lines = {}
     for name, prec, window in (("fp64", X.XS_FP64, 0), ("fp32", X.XS_FP32, 0),
+                               ("fp64_one_window", X.XS_FP64, -1),
+                               ("fp32_one_window", X.XS_FP32, -1),
                                ("fp32_window16M", X.XS_FP32, 16_000_000)):
         plan = {}
         dev.batched(sizes, seeds, thr, precision=prec, mode=0, window=window, plan=plan)  # tunes

@@ -128,9 +128,13 @@ typedef struct {
  * a key kernel, an LSD radix sort of all lookups by (sort window, material,
  * energy), then one persistent lookup kernel whose workgroups take tiles of
  * the sorted order from per-XCD work queues (stealing across XCDs when
- * their own queue runs dry).  Batches are grouped in order into sort windows
- * of about `window` lookups; windows run in order, so a smaller window trades
- * throughput for earlier completion of the first batches.  Zero fields of
+ * their own queue runs dry).  Batches are grouped into sort windows that
+ * run one after the other: by default (window 0) the smallest batches -- at
+ * most 55% of them, holding at most 20% of the lookups -- form a first
+ * window, so that most batches complete after a small part of the work;
+ * window > 0 groups the batches in order into windows of about `window`
+ * lookups; window -1 runs all batches in one window (the highest throughput,
+ * every batch completing at the end).  Zero fields of
  * the plan (tile, stage, sort, mapping) are chosen by an autotuner keyed on the
  * nuclide count, grid type, precision and batch-size histogram, whose choice
  * is cached on the handle. */
@@ -160,7 +164,8 @@ typedef struct {
 	int32_t tile;              /* lookups per work item (0 = autotuned in mode 0, 4096 in mode 2) */
 	int32_t stage;             /* mode 0: 8 or 16 (0 = autotuned) */
 	int32_t sort;              /* mode 0: 1 or 2 (0 = autotuned) */
-	int64_t window;            /* mode 0: lookups per sort window (0 = one window) */
+	int64_t window;            /* mode 0: 0 small batches first (two windows), > 0 about this
+	                              many lookups per window in batch order, -1 one window */
 	int32_t eager;             /* mode 0: 1 = launch the kernels directly instead of the graph */
 	int32_t mapping;           /* mode 0: 1 lane per lookup, 2 wave-uniform, 3 wave-uniform
 	                              without the two-interval step (0 = autotuned) */

@@ -564,12 +564,21 @@ def test_batched_workload_large_against_oracle(orc, prec):
     want = _BATCH_TRUTH["want"]
     plan = {}
     r, vs, done = dev.batched(sizes, seeds, thr, precision=prec, mode=0, plan=plan)
-    print(f"[gpu-test] batched large prec={prec}: {r['ms_total']:.2f} ms, plan {plan}", flush=True)
+    print(f"[gpu-test] batched large prec={prec}: {r['ms_total']:.2f} ms, plan {plan}, "
+          f"completion p50 {np.percentile(done, 50):.2f} ms p99 {np.percentile(done, 99):.2f} ms",
+          flush=True)
     assert [int(v) for v in vs] == want
     assert r["vsum"] == sum(want)
+    # the default windows: the smallest batches first, so most complete early
+    assert plan["windows"] == 2
+    assert np.percentile(done, 50) < 0.5 * np.percentile(done, 99)
     # the replayed graph of the tuned plan, with the 16-bit index tables it uses
     r2, vs2, _ = dev.batched(sizes, seeds, thr, precision=prec, mode=0)
     assert [int(v) for v in vs2] == want
+    # one window: every batch completes at the end, the same sums
+    p1 = {}
+    _, vs3, done3 = dev.batched(sizes, seeds, thr, precision=prec, mode=0, window=-1, plan=p1)
+    assert p1["windows"] == 1 and [int(v) for v in vs3] == want
 
 
 @pytest.mark.parametrize("name", ["small_unionized", "small_hash", "small_nuclide"])
@@ -840,3 +849,30 @@ def test_table_bytes_predicts_the_upload(name):
             assert got[k] == v, k
     if inp.grid_type == X.UNIONIZED:
         assert got["bytes_index"] == 0
+
+
+def test_xl_unionized_fits_and_matches_the_hash_grid():
+    """the XL problem (355 x 238,847 points) on the unionized grid, built on
+    the device: about 186 GB of tables, int32 per-material index tables and no
+    whole index grid; its verification sum equals the hash grid's on the same
+    lookups (the reference's grid-type invariance, README.md:164).  Runs last:
+    it first frees the session's other device copies."""
+    for name in list(_CACHE):
+        _CACHE.pop(name)[2].close()
+    n = 2_000_000
+    want = {}
+    for grid in ("hash", "unionized"):
+        inp = X.make_inputs("XL", grid, "event")
+        print(f"[gpu-test] XL {grid}: building on the device", flush=True)
+        with X.Device(inp, None) as dev:
+            info = dev.info()
+            if grid == "unionized":
+                assert info["bytes_index"] == 0
+                assert info == {**info, **{k: v for k, v in X.table_bytes(inp).items()
+                                           if k.startswith("bytes_")}}
+            r = dev.event(n, 7, kernel_id=X.K_MAT_E_SORT)
+            r0 = dev.event(n // 8, 7, kernel_id=X.K_BASELINE)
+            print(f"[gpu-test] XL {grid}: {sum(v for k, v in info.items() if k.startswith('bytes_')) / 1e9:.1f} GB "
+                  f"of tables, {r['ms_total']:.2f} ms per {n} lookups", flush=True)
+            want[grid] = (r["vsum"], r0["vsum"])
+    assert want["hash"] == want["unionized"]

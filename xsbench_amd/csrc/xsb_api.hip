@@ -626,7 +626,10 @@ static int build_material_index(Device *d, const xsb_tables *t, bool from_host, 
 		return fail(XSB_ERR_OOM, "hipMalloc of %zu bytes (material index tables) failed", bytes);
 	}
 	d->bytes_midx = (int64_t)bytes;
-	const int64_t chunk = std::max<int64_t>(1, kIndexChunkBytes / (n_iso * 4));
+	// device sweeps take larger chunks: a sweep block walks 2048 rows, so a
+	// chunk of 64 MB would leave most of the device idle
+	const int64_t chunk_bytes = from_host ? kIndexChunkBytes : 8 * kIndexChunkBytes;
+	const int64_t chunk = std::max<int64_t>(1, chunk_bytes / (n_iso * 4));
 	void *stage = nullptr;
 	int rc = scratch_reserve(d, 0, (size_t)std::min(rows, chunk) * n_iso * 4, &stage);
 	if (rc)

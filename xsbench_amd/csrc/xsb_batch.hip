@@ -11,14 +11,18 @@
 //   k_bkeys     key = window << (4 + eb) | mat << eb | top eb bits of E, and
 //               the lookup id, for every lookup of every batch
 //   radix sort  (window, material, energy) order, ids as payload
-//   k_bsorted   persistent lookup kernel: workgroups take tiles of the
-//               sorted order from 8 per-XCD queues (the XCD's own slice of
-//               the order first, so its L2 sees neighbouring energies, then
-//               the other queues; with several sort windows the slices are
-//               dealt out in short runs so windows finish in order); per-batch
-//               sums and counts are gathered in
-//               LDS and flushed with one atomic per batch; a batch's
-//               completion time is taken when its count reaches its size.
+//   k_bsorted   persistent lookup kernel: sort window after sort window,
+//               workgroups take tiles of the window's part of the sorted
+//               order from 8 per-XCD queues (the XCD's own slice first, so
+//               its L2 sees neighbouring energies, then the other queues);
+//               per-batch sums and counts are gathered in LDS and flushed
+//               with one atomic per batch (after every tile when there are
+//               several windows); a batch's completion time is taken when
+//               its count reaches its size.
+// Sort windows: by default the smallest batches (at most 55% of them and 20%
+// of the lookups) form a first window and the rest a second, so that most
+// batches complete after a small part of the work; or windows of about
+// `window` lookups in batch order; or one window.
 // mode 1: one fused launch per batch, in batch order.
 // mode 2: one persistent kernel over a queue of tiles in lookup order; each
 //         tile is ordered by material inside the workgroup.
@@ -237,10 +241,8 @@ static void launch_one(Device *d, int64_t size, uint64_t seed, const double *thr
 // ---- mode 0: sorted pipeline ----
 constexpr int kLdsBatches = 1024;  // per-batch LDS sums when n_batches <= this
 constexpr int kXcds = 8;
-constexpr int kRunItems = 4;       // consecutive items per XCD turn with several windows
-
 // Batch tables on the device.  off[b] = first global id of batch b,
-// off[nb] = total; win[b] = sort window of batch b (non-decreasing).
+// off[nb] = total; win[b] = sort window of batch b.
 struct BatchView {
 	const int64_t *sizes;
 	const uint64_t *seeds;
@@ -269,27 +271,31 @@ struct SortedQueue {
 	const uint2 *pairs;        // sorted {key, global lookup id}: material in key
 	                           // bits [ebits, ebits + 4)
 	int64_t n;
-	int64_t n_items;
+	const uint32_t *wpos;      // [windows + 1]: window w holds sorted positions
+	                           // [wpos[w], wpos[w + 1])
+	int32_t windows;
 	int32_t ebits;
 	int32_t tile;
 	int32_t flush_each_item;   // several sort windows: flush LDS sums after every item
 	int32_t wave;              // wave-uniform lookup path for material-uniform waves
-	uint32_t *heads;           // [8] items taken from each XCD's slice
+	int32_t reserved0;
+	uint32_t *heads;           // [windows x 8] items taken from each (window, XCD) slice
 	unsigned long long *vsum;  // [nb]
 	unsigned long long *cnt;   // [nb] lookups completed
 	uint64_t *done_clock;      // [nb]
 	unsigned long long *nfb;   // [1]
 };
 
-__global__ void k_breset(SortedQueue q, int nb)
+__global__ void k_breset(SortedQueue q, int nb, int windows)
 {
 	for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
 		q.vsum[b] = 0;
 		q.cnt[b] = 0;
 		q.done_clock[b] = 0;
 	}
-	if (blockIdx.x == 0 && threadIdx.x < kXcds)
-		q.heads[threadIdx.x] = 0;
+	for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < windows * kXcds;
+	     h += gridDim.x * blockDim.x)
+		q.heads[h] = 0;
 	if (blockIdx.x == 0 && threadIdx.x == 0)
 		*q.nfb = 0;
 }
@@ -345,8 +351,12 @@ __device__ __forceinline__ void flush_batches(const BatchView &B, const SortedQu
 	}
 }
 
+// At least four waves per SIMD (at most 128 VGPRs): left free, the register
+// allocator lands on either side of 128 from one variant to the next, and
+// three waves lose about a quarter of the throughput.
 template <int GRID, int PREC, int CHUNK, bool SHARE3>
-__global__ __launch_bounds__(kBatchBlock) void k_bsorted(Tables T, BatchView B, SortedQueue q)
+__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_bsorted(
+    Tables T, BatchView B, SortedQueue q)
 {
 	__shared__ uint32_t s_off[kLdsBatches + 1];
 	__shared__ uint32_t s_v[kLdsBatches], s_c[kLdsBatches];
@@ -363,30 +373,34 @@ __global__ __launch_bounds__(kBatchBlock) void k_bsorted(Tables T, BatchView B, 
 	const uint32_t *off = lds ? s_off : B.off;
 	uint32_t fb_total = 0, since_flush = 0;
 	// the dispatcher places workgroup w on XCD w % 8
-	int xq = (int)(blockIdx.x % kXcds);
-	for (int tries = 0; tries < kXcds;) {
-		if (threadIdx.x == 0)
-			s_item = (int)atomicAdd(q.heads + xq, 1u);
-		__syncthreads();
-		// queue xq's k-th item: one slice of the sorted order per XCD, or, with
-		// several sort windows, runs of kRunItems items dealt to the XCDs in
-		// turn, so that all XCDs finish window w before window w + 1
-		const int64_t k = s_item;
-		int64_t item, end;
-		if (q.flush_each_item) {
-			item = (k / kRunItems) * (kXcds * kRunItems) + xq * kRunItems + (k % kRunItems);
-			end = q.n_items;
-		} else {
-			item = q.n_items * xq / kXcds + k;
-			end = q.n_items * (xq + 1) / kXcds;
+	const int home = (int)(blockIdx.x % kXcds);
+	// window after window; each window's tiles are dealt in 8 contiguous
+	// slices, one per XCD's queue; a workgroup drains its own XCD's queue,
+	// then the others', then moves on to the next window (every workgroup
+	// reaches the end: each queue hands out its items once)
+	for (int w = 0, xq = home, tries = 0; w < q.windows;) {
+		if (tries == kXcds) {
+			w++;
+			xq = home;
+			tries = 0;
+			continue;
 		}
+		// the queue position is read once per workgroup through LDS and kept
+		// wave-uniform (scalar registers) from there on
+		const int64_t lo = q.wpos[w], hi = q.wpos[w + 1];
+		const int64_t n_items = (hi - lo + q.tile - 1) / q.tile;
+		if (threadIdx.x == 0)
+			s_item = (int)atomicAdd(q.heads + w * kXcds + xq, 1u);
+		__syncthreads();
+		const int64_t item = n_items * xq / kXcds + __builtin_amdgcn_readfirstlane(s_item);
+		const int64_t end = n_items * (xq + 1) / kXcds;
 		__syncthreads();  // every thread has read s_item before it is rewritten
 		if (item >= end) {  // this queue is drained: move on to the next one
 			xq = (xq + 1) % kXcds;
 			tries++;
 			continue;
 		}
-		const int64_t p0 = item * q.tile, p1 = min(p0 + (int64_t)q.tile, q.n);
+		const int64_t p0 = lo + item * q.tile, p1 = min(p0 + (int64_t)q.tile, hi);
 		// all threads run the same trip count; lanes past p1 repeat p1 - 1 and
 		// contribute nothing, so waves stay whole for the wave-wide paths
 		for (int64_t r = p0; r < p1; r += kBatchBlock) {
@@ -565,21 +579,27 @@ static int ilog2_ceil(uint64_t x)
 }
 
 // Device layout of the batch tables in scratch slot 1.
+// wpos: the windows' first sorted positions (at most one window per
+// batch); wpos1: the single window of the tuner's runs; heads: one queue
+// head per (window, XCD).
 struct TableLayout {
-	size_t sizes, seeds, thr, off, win, vsum, cnt, done, heads, nfb, clk, bytes;
+	size_t sizes, seeds, thr, off, win, wpos, wpos1, vsum, cnt, done, heads, nfb, clk, bytes;
 	explicit TableLayout(int nb)
 	{
 		auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+		const size_t nw = (size_t)std::max(nb, 1);
 		sizes = 0;
 		seeds = up(sizes + 8 * (size_t)nb);
 		thr = up(seeds + 8 * (size_t)nb);
 		off = up(thr + 8 * (size_t)kMaterials * nb);
 		win = up(off + 4 * ((size_t)nb + 1));
-		vsum = up(win + 4 * (size_t)nb);
+		wpos = up(win + 4 * (size_t)nb);
+		wpos1 = up(wpos + 4 * (nw + 1));
+		vsum = up(wpos1 + 4 * 2);
 		cnt = up(vsum + 8 * (size_t)nb);
 		done = up(cnt + 8 * (size_t)nb);
 		heads = up(done + 8 * (size_t)nb);
-		nfb = up(heads + 4 * kXcds);
+		nfb = up(heads + 4 * kXcds * nw);
 		clk = up(nfb + 8);
 		bytes = up(clk + 8 * 4);
 	}
@@ -593,7 +613,8 @@ static int enqueue_pipeline(Device *d, const PipeShape &ps, const BatchView &B, 
 	uint2 *pA = (uint2 *)ps.sortbuf, *pB = pA + n2;
 	uint32_t *hist = (uint32_t *)(pB + n2), *part = hist + sort_hist_len(ps.n);
 	const int ntiles = sort_tiles(ps.n);
-	hipLaunchKernelGGL(k_breset, dim3((ps.nb + 255) / 256), dim3(256), 0, s, q0, ps.nb);
+	hipLaunchKernelGGL(k_breset, dim3((std::max(ps.nb, ps.windows * kXcds) + 255) / 256), dim3(256),
+	                   0, s, q0, ps.nb, ps.windows);
 	hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, clk);
 	hipLaunchKernelGGL(k_bkeys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, B, ps.n, ps.ebits, pA,
 	                   hist, ntiles);
@@ -605,8 +626,8 @@ static int enqueue_pipeline(Device *d, const PipeShape &ps, const BatchView &B, 
 	SortedQueue q = q0;
 	q.pairs = pA;
 	q.n = ps.n;
+	q.windows = ps.windows;
 	q.tile = ps.plan.tile;
-	q.n_items = (ps.n + ps.plan.tile - 1) / ps.plan.tile;
 	q.ebits = ps.ebits;
 	q.flush_each_item = ps.windows > 1;
 	q.wave = ps.plan.mapping >= 2;
@@ -631,7 +652,9 @@ static int make_shape(Device *d, int nb, int64_t n, int windows, int prec, const
 	const int wbits = windows > 1 ? ilog2_ceil((uint64_t)windows) : 0;
 	if (wbits > 26)
 		return fail(XSB_ERR_INVALID, "%d sort windows is more than 2^26", windows);
-	ps->ebits = plan.sort == 2 ? std::min(16, 26 - wbits) : 0;
+	// two radix passes (20 key bits) while the energy keeps 14 bits or more;
+	// beyond that a third pass, for 16 energy bits
+	ps->ebits = plan.sort != 2 ? 0 : (wbits <= 2 ? 16 - wbits : std::min(16, 26 - wbits));
 	const int bits = wbits + 4 + ps->ebits;
 	ps->passes = (bits + kDigitBits - 1) / kDigitBits;
 	int occ = 0;
@@ -648,7 +671,7 @@ static int make_shape(Device *d, int nb, int64_t n, int windows, int prec, const
 }
 
 static void make_views(const PipeShape &ps, char *base, const TableLayout &L, BatchView *B,
-                       SortedQueue *q)
+                       SortedQueue *q, bool tuning = false)
 {
 	B->sizes = (const int64_t *)(base + L.sizes);
 	B->seeds = (const uint64_t *)(base + L.seeds);
@@ -657,6 +680,7 @@ static void make_views(const PipeShape &ps, char *base, const TableLayout &L, Ba
 	B->win = (const uint32_t *)(base + L.win);
 	B->nb = ps.nb;
 	memset(q, 0, sizeof(*q));
+	q->wpos = (const uint32_t *)(base + (tuning ? L.wpos1 : L.wpos));
 	q->heads = (uint32_t *)(base + L.heads);
 	q->vsum = (unsigned long long *)(base + L.vsum);
 	q->cnt = (unsigned long long *)(base + L.cnt);
@@ -666,16 +690,16 @@ static void make_views(const PipeShape &ps, char *base, const TableLayout &L, Ba
 
 // Runs the pipeline once, eagerly, over the first nb batches; returns the
 // GPU time in *ms.
-static int time_pipeline(Device *d, int nb, int64_t n, int windows, int prec, const BatchPlan &plan,
-                         char *base, const TableLayout &L, float *ms)
+static int time_pipeline(Device *d, int nb, int64_t n, int prec, const BatchPlan &plan, char *base,
+                         const TableLayout &L, float *ms)
 {
 	PipeShape ps;
-	int rc = make_shape(d, nb, n, windows, prec, plan, base, &ps);
+	int rc = make_shape(d, nb, n, 1, prec, plan, base, &ps);
 	if (rc)
 		return rc;
 	BatchView B;
 	SortedQueue q;
-	make_views(ps, base, L, &B, &q);
+	make_views(ps, base, L, &B, &q, true);
 	hipStream_t s = d->stream;
 	XSB_HIP(hipEventRecord(d->ev[4], s));
 	if ((rc = enqueue_pipeline(d, ps, B, q, (uint64_t *)(base + L.clk), s)))
@@ -687,12 +711,12 @@ static int time_pipeline(Device *d, int nb, int64_t n, int windows, int prec, co
 }
 
 // Picks the plan fields left open in `fixed` by timing each candidate (twice,
-// keeping the faster) on a prefix of the batches of about 16 M lookups; a
-// prefix of a random set of batches keeps its size histogram.  The staging
-// depth only matters on the unionized grid.
+// keeping the faster) on a prefix of the batches of about 16 M lookups in one
+// sort window (wpos1 = {0, its lookups}); a prefix of a random set of
+// batches keeps its size histogram.  The staging depth only matters on the
+// unionized grid.
 static int autotune(Device *d, const xsb_batched_args *a, const std::vector<uint32_t> &off,
-                    const std::vector<uint32_t> &win, char *base, const TableLayout &L,
-                    const BatchPlan &fixed, BatchPlan *plan)
+                    char *base, const TableLayout &L, const BatchPlan &fixed, BatchPlan *plan)
 {
 	const int nb = a->n_batches;
 	int nbt = nb;
@@ -702,7 +726,8 @@ static int autotune(Device *d, const xsb_batched_args *a, const std::vector<uint
 			break;
 		}
 	const int64_t nt = off[nbt];
-	const int wt = (int)win[nbt - 1] + 1;
+	const uint32_t w1[2] = {0u, (uint32_t)nt};
+	XSB_HIP(hipMemcpy(base + L.wpos1, w1, sizeof(w1), hipMemcpyHostToDevice));
 	auto pick = [](int v, std::vector<int> all) { return v ? std::vector<int>{v} : all; };
 	const bool staged = d->T.grid_type == kUnionized;
 	const std::vector<int> tiles = pick(fixed.tile, {2048, 8192});
@@ -718,9 +743,9 @@ static int autotune(Device *d, const xsb_batched_args *a, const std::vector<uint
 				for (int st : stages) {
 					const BatchPlan c{ti, st, so, mp};
 					float ms = 0, ms2 = 0;
-					int rc = time_pipeline(d, nbt, nt, wt, a->xs_precision, c, base, L, &ms);
+					int rc = time_pipeline(d, nbt, nt, a->xs_precision, c, base, L, &ms);
 					if (!rc)
-						rc = time_pipeline(d, nbt, nt, wt, a->xs_precision, c, base, L, &ms2);
+						rc = time_pipeline(d, nbt, nt, a->xs_precision, c, base, L, &ms2);
 					if (rc)
 						return rc;
 					ms = std::min(ms, ms2);
@@ -750,6 +775,7 @@ static int run_sorted_batches(Device *d, const xsb_batched_args *a, xsb_result *
 	off[nb] = (uint32_t)total;
 	int windows = 1;
 	if (a->window > 0) {
+		// windows of about a->window lookups, in batch order
 		int64_t last = -1;
 		int w = -1;
 		for (int b = 0; b < nb; b++) {
@@ -761,7 +787,36 @@ static int run_sorted_batches(Device *d, const xsb_batched_args *a, xsb_result *
 			win[b] = (uint32_t)w;
 		}
 		windows = w + 1;
+	} else if (a->window == 0 && nb > 1) {
+		// the smallest batches first: at most 55% of them, holding at most 20%
+		// of the lookups, in a window of their own
+		std::vector<int> order(nb);
+		for (int b = 0; b < nb; b++)
+			order[b] = b;
+		std::stable_sort(order.begin(), order.end(),
+		                 [&](int x, int y) { return a->sizes[x] < a->sizes[y]; });
+		int64_t small = 0;
+		int taken = 0;
+		for (int i = 0; i < nb && 100 * (i + 1) <= 55 * nb; i++) {
+			if (5 * (small + a->sizes[order[i]]) > total)
+				break;
+			small += a->sizes[order[i]];
+			taken++;
+		}
+		if (taken > 0 && taken < nb) {
+			for (int b = 0; b < nb; b++)
+				win[b] = 1;
+			for (int i = 0; i < taken; i++)
+				win[order[i]] = 0;
+			windows = 2;
+		}
 	}
+	// window w holds sorted positions [wpos[w], wpos[w + 1])
+	std::vector<uint32_t> wpos(windows + 1, 0);
+	for (int b = 0; b < nb; b++)
+		wpos[win[b] + 1] += (uint32_t)a->sizes[b];
+	for (int w = 0; w < windows; w++)
+		wpos[w + 1] += wpos[w];
 	hipStream_t s = d->stream;
 	if (!d->batch)
 		d->batch = new BatchState();
@@ -780,6 +835,8 @@ static int run_sorted_batches(Device *d, const xsb_batched_args *a, xsb_result *
 	                       hipMemcpyHostToDevice, s));
 	XSB_HIP(hipMemcpyAsync(base + L.off, off.data(), 4 * ((size_t)nb + 1), hipMemcpyHostToDevice, s));
 	XSB_HIP(hipMemcpyAsync(base + L.win, win.data(), 4 * (size_t)nb, hipMemcpyHostToDevice, s));
+	XSB_HIP(hipMemcpyAsync(base + L.wpos, wpos.data(), 4 * ((size_t)windows + 1),
+	                       hipMemcpyHostToDevice, s));
 	XSB_HIP(hipStreamSynchronize(s));  // host vectors go out of scope with pending copies otherwise
 
 	// plan: the caller's fields, the rest tuned (cached per key)
@@ -804,7 +861,7 @@ static int run_sorted_batches(Device *d, const xsb_batched_args *a, xsb_result *
 			}
 		if (!found) {
 			const auto t0 = std::chrono::steady_clock::now();
-			if ((rc = autotune(d, a, off, win, base, L, fixed, &plan)))
+			if ((rc = autotune(d, a, off, base, L, fixed, &plan)))
 				return rc;
 			ms_tuning = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 			tuned = 1;
@@ -1020,8 +1077,8 @@ int run_batched(Device *d, const xsb_batched_args *a, xsb_result *r)
 		return fail(XSB_ERR_INVALID, "n_batches %d is more than 2^20", a->n_batches);
 	if (a->mode < 0 || a->mode > 2)
 		return fail(XSB_ERR_INVALID, "mode %d is not 0, 1 or 2", a->mode);
-	if (a->tile < 0 || a->stage < 0 || a->sort < 0 || a->sort > 2 || a->window < 0)
-		return fail(XSB_ERR_INVALID, "tile, stage, window must be >= 0 and sort 0, 1 or 2");
+	if (a->tile < 0 || a->stage < 0 || a->sort < 0 || a->sort > 2 || a->window < -1)
+		return fail(XSB_ERR_INVALID, "tile, stage must be >= 0, window >= -1 and sort 0, 1 or 2");
 	const int nb = a->n_batches;
 	for (int b = 0; b < nb; b++) {
 		if (a->sizes[b] < 0 || a->sizes[b] > ((int64_t)1 << 31))
