@@ -195,8 +195,12 @@ typedef struct {
 	int64_t bytes_fp64;       /* FP64 AoS grid */
 	int64_t bytes_fp32;       /* FP32 XS table (0 if not built) */
 	int64_t bytes_bf16;       /* BF16 XS table (0 if not built) */
-	int64_t bytes_index;      /* hash index grid (rows padded to 16 bytes); the unionized index
-	                             grid is not kept whole on the device (0) */
+	int64_t bytes_index;      /* hash grid search tables: the index grid (int32 rows padded to
+	                             16 bytes), 96-byte window records per bin and nuclide (same
+	                             rows: u_low, u_high and ten energies) and the nuclides'
+	                             energies by row (n_gridpoints + 16 rounded up to 8 doubles
+	                             each); the unionized index grid is not kept whole on the
+	                             device (0) */
 	int64_t bytes_midx;       /* per-material copies of the unionized index grid */
 	int64_t bytes_union;      /* unionized energies */
 	int64_t bytes_ivl;        /* interval records of the unionized grid: FP64, plus FP32 with
@@ -289,7 +293,7 @@ int xsb_hip_balg(void *handle, const xsb_event_args *args, double *bytes_per_loo
  * so bytes / lookup-kernel time / HBM peak is a roofline fraction of at most
  * 1 (up to lines still resident in the last-level cache from earlier work).
  * The FP64 re-evaluations of mixed-precision lookups are not counted. */
-#define XSB_LINE_REGIONS 9
+#define XSB_LINE_REGIONS 11
 typedef struct {
 	int64_t bytes;           /* bytes_tables + bytes_schedule + bytes_small */
 	int64_t bytes_tables;    /* lines of the lookup tables (sum of region_bytes) */
@@ -298,7 +302,8 @@ typedef struct {
 	int64_t launches;        /* lookup-kernel launches the bytes spread over (history: rounds) */
 	/* per table: unionized energies, unionized search brackets, per-material
 	 * index tables, FP64 interval records, FP32 interval records, hash index
-	 * grid, FP64 points, FP32 records, BF16 records */
+	 * grid, FP64 points, FP32 records, BF16 records, hash window records,
+	 * hash energy rows */
 	int64_t region_bytes[XSB_LINE_REGIONS];
 } xsb_lines;
 

@@ -147,6 +147,44 @@ def test_line_count_hash_matches_walk():
     assert got["bytes_tables"] == w["index"] + w["grid"]
 
 
+def _walk_hash_history(inp, sd, E, m):
+    """per-region line bytes of the history rounds' hash lookups (E, m): the
+    bin's 96-byte window record (index words and ten energies), eight more
+    energies of the nuclide's row where the bin spans further, or its
+    bisection over the row, and the two bracket points"""
+    a = X.sd_arrays(inp, sd)
+    n_iso, n_gp, bins = inp.n_isotopes, inp.n_gridpoints, inp.hash_bins
+    g = a["nuclide_grid"].reshape(n_iso, n_gp, 6)
+    IG = a["index_grid"].reshape(-1, n_iso)
+    pitch = (n_iso + 3) // 4 * 4
+    e_pitch = (n_gp + 16 + 7) // 8 * 8
+    nn, mats = a["num_nucs"], a["mats"]
+    win, row, pts = [], [], []
+    for p, mat in zip(E, m):
+        b = min(int(p / (1.0 / bins)), bins - 1)
+        for j in range(int(nn[mat])):
+            nuc = int(mats[mat, j])
+            w = 96 * (b * pitch + nuc)
+            win.append((w, w + 95))
+            u_low, u_high, lo = _bracket_hash(g, nuc, n_gp, IG, b, bins, p)
+            base, top = nuc * e_pitch * 8, u_high - (u_low & ~1)
+            En = g[nuc, :, 0]
+            if top > 17:
+                row.append((base + 8 * u_high, base + 8 * u_high + 7))
+                if En[u_low] < p < En[u_high]:
+                    x, y = u_low, u_high
+                    while y - x > 1:
+                        mid = x + (y - x) // 2
+                        row.append((base + 8 * mid, base + 8 * mid + 7))
+                        x, y = (x, mid) if En[mid] > p else (mid, y)
+            elif top > 9:
+                s0 = (u_low & ~1) + 10
+                row.append((base + 8 * s0, base + 8 * s0 + 63))
+            pb = nuc * n_gp * 48
+            pts.append((pb + 48 * lo, pb + 48 * lo + 95))
+    return {"hwin": _lines(win), "esoa": _lines(row), "grid": _lines(pts)}
+
+
 def test_line_count_history_matches_walk(orc):
     """round r's lines are those of lookup r of every particle: the chains
     replayed on the host (the reference's LCG, pick_mat and fast-forward,
@@ -165,8 +203,8 @@ def test_line_count_history_matches_walk(orc):
     for r in range(R):
         E = np.array([lib.LCG_random_double(C.byref(s)) for s in seeds])
         m = np.array([lib.pick_mat(C.byref(s)) for s in seeds], np.int32)
-        w = _walk_hash(inp, sd, E, m)
-        want += w["index"] + w["grid"]
+        w = _walk_hash_history(inp, sd, E, m)
+        want += w["hwin"] + w["esoa"] + w["grid"]
         macro, _ = orc.lookup_detail(oi, osd, E, m)
         for q, row in enumerate(macro):
             nf = int((row > 1.0).sum())

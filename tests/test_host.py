@@ -450,4 +450,6 @@ def test_table_sizes_fit_one_gpu():
     # uint16 entries: rows of 328, 8, 8, 8, 32, 24 x 5, 16, 16 entries
     assert lg["bytes_index"] == 0 and lg["bytes_midx"] == 355 * 11303 * 2 * (328 + 8 * 3 + 32 + 24 * 5 + 16 * 2)
     hs = X.table_bytes(X.make_inputs("large", "hash", "event"))
-    assert hs["bytes_index"] == 10000 * 356 * 4 and hs["bytes_midx"] == 0
+    # the int32 index grid and the 96-byte window records, rows of 356 entries;
+    # the energy rows, 11303 + 16 rounded up to 11320 doubles
+    assert hs["bytes_index"] == 10000 * 356 * (4 + 96) + 355 * 11320 * 8 and hs["bytes_midx"] == 0

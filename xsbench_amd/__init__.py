@@ -154,7 +154,8 @@ class XsbResult(C.Structure):
 
 
 # tables of xsb_lines.region_bytes, in order (XSB_LINE_REGIONS)
-LINE_REGIONS = ("ueg", "acc", "midx", "ivl", "ivl32", "index", "grid", "xs32", "xs16")
+LINE_REGIONS = ("ueg", "acc", "midx", "ivl", "ivl32", "index", "grid", "xs32", "xs16", "hwin",
+                "esoa")
 N_LINE_REGIONS = len(LINE_REGIONS)
 
 

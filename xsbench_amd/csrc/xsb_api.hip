@@ -114,9 +114,9 @@ static void release(Device *d)
 	batch_state_free(d);
 	if (d->history_graph.exec)
 		(void)hipGraphExecDestroy(d->history_graph.exec);
-	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_midx, d->d_grid,  d->d_ivl, d->d_ivl32, d->d_ueg_acc, d->d_xs32, d->d_xs16,
-	                d->d_xs16_scale, d->d_num_nucs, d->d_mats, d->d_concs, d->d_concs32, d->d_acc,
-	                d->d_fftab};
+	void *ptrs[] = {d->d_ueg,  d->d_index, d->d_hwin, d->d_esoa, d->d_midx, d->d_grid,  d->d_ivl,
+	                d->d_ivl32, d->d_ueg_acc, d->d_xs32, d->d_xs16, d->d_xs16_scale, d->d_num_nucs,
+	                d->d_mats, d->d_concs, d->d_concs32, d->d_acc, d->d_fftab};
 	for (void *p : ptrs)
 		if (p)
 			(void)hipFree(p);
@@ -293,6 +293,79 @@ __global__ void k_build_bf16(const double *__restrict__ g, int64_t n, int32_t n_
 		*reinterpret_cast<uint2 *>(r + 1) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
 		*reinterpret_cast<uint2 *>(r + 2) = make_uint2(h[4], 0u);
 	}
+}
+
+// Hash grid search tables (Tables::esoa, Tables::hwin).
+__global__ void k_energy_rows(const double *__restrict__ grid, int32_t n_iso, int32_t n_gp,
+                              int32_t e_pitch, double *__restrict__ esoa)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)n_iso * e_pitch;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const int64_t nuc = i / e_pitch;
+		const int32_t k = (int32_t)(i - nuc * e_pitch);
+		esoa[i] = k < n_gp ? grid[(nuc * n_gp + k) * kPointDoubles] : (double)INFINITY;
+	}
+}
+
+// Index entries are in [0, n_gp - 2] (checked before), so u_high <= n_gp - 1
+// and the window E[b .. b + 9] lies inside the energy row's pitch.
+__global__ void k_hash_windows(const int32_t *__restrict__ index, int64_t bins, int32_t n_iso,
+                               int32_t pitch, int32_t n_gp, const double *__restrict__ esoa,
+                               int32_t e_pitch, double *__restrict__ hwin)
+{
+	for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < bins * pitch;
+	     i += (int64_t)gridDim.x * blockDim.x) {
+		const int64_t b = i / pitch;
+		const int32_t nuc = (int32_t)(i - b * pitch);
+		double *r = hwin + i * kHwinDoubles;
+		if (nuc >= n_iso) {
+			for (int k = 0; k < kHwinDoubles; k++)
+				r[k] = 0.0;
+			continue;
+		}
+		const int32_t u_low = index[i];
+		const int32_t u_high = (b == bins - 1) ? n_gp - 1 : index[i + pitch] + 1;
+		r[0] = __hiloint2double(u_high, u_low);
+		r[1] = 0.0;
+		const double *E = esoa + (int64_t)nuc * e_pitch + (u_low & ~1);
+		for (int k = 0; k < 10; k++)
+			r[2 + k] = E[k];
+	}
+}
+
+// Row pitch of Tables::esoa: room for an eighteen-energy window from any
+// bin's u_low <= n_gp - 2, rounded down to even, and 64-byte aligned rows.
+static int64_t energy_pitch(int64_t n_gp)
+{
+	return (n_gp + 16 + 7) & ~(int64_t)7;
+}
+
+static int build_hash_search(Device *d, int64_t n_iso, int64_t n_gp, int64_t bins, int64_t pitch)
+{
+	const int64_t e_pitch = energy_pitch(n_gp);
+	const size_t wb = (size_t)bins * pitch * kHwinDoubles * 8, eb = (size_t)n_iso * e_pitch * 8;
+	if (hipMalloc((void **)&d->d_esoa, eb) != hipSuccess) {
+		(void)hipGetLastError();
+		d->d_esoa = nullptr;
+		return fail(XSB_ERR_OOM, "hipMalloc of %zu bytes (hash energy rows) failed", eb);
+	}
+	if (hipMalloc((void **)&d->d_hwin, wb) != hipSuccess) {
+		(void)hipGetLastError();
+		d->d_hwin = nullptr;
+		return fail(XSB_ERR_OOM, "hipMalloc of %zu bytes (hash window records) failed", wb);
+	}
+	hipLaunchKernelGGL(k_energy_rows, dim3(4096), dim3(256), 0, d->stream, d->d_grid, (int32_t)n_iso,
+	                   (int32_t)n_gp, (int32_t)e_pitch, d->d_esoa);
+	XSB_HIP(hipGetLastError());
+	hipLaunchKernelGGL(k_hash_windows, dim3(4096), dim3(256), 0, d->stream, d->d_index, bins,
+	                   (int32_t)n_iso, (int32_t)pitch, (int32_t)n_gp, d->d_esoa, (int32_t)e_pitch,
+	                   d->d_hwin);
+	XSB_HIP(hipGetLastError());
+	d->bytes_index += (int64_t)(wb + eb);
+	d->T.hwin = d->d_hwin;
+	d->T.esoa = d->d_esoa;
+	d->T.e_pitch = (int32_t)e_pitch;
+	return XSB_OK;
 }
 
 // Per-nuclide max |xs| over the points and channels of host tables.
@@ -731,7 +804,12 @@ static void table_bytes(const xsb_tables *t, uint32_t flags, xsb_info *o)
 	o->bytes_fp32 = (flags & XSB_BUILD_FP32) ? n * kRec32Doubles * 8 : 0;
 	o->bytes_bf16 = (flags & XSB_BUILD_BF16) ? n * kRec16Doubles * 8 : 0;
 	o->bytes_union = uni ? n * 8 : 0;
-	o->bytes_index = (t->grid_type == XSB_HASH) ? (int64_t)t->hash_bins * ((n_iso + 3) & ~3) * 4 : 0;
+	// hash grid: the index grid (int32, rows padded to 16 bytes), its window
+	// records (same rows) and the energy rows (Tables::hwin, ::esoa)
+	o->bytes_index = (t->grid_type == XSB_HASH) ? (int64_t)t->hash_bins * ((n_iso + 3) & ~3) *
+	                                                      (4 + kHwinDoubles * 8) +
+	                                                  n_iso * energy_pitch(n_gp) * 8
+	                                            : 0;
 	if (uni) {
 		const bool narrow = !(flags & XSB_BUILD_IDX32) && n_gp <= 65536;
 		const int align = narrow ? 8 : 4, width = narrow ? 2 : 4;
@@ -878,6 +956,8 @@ static int upload(int device, const xsb_tables *t, uint32_t flags, Device **out)
 	T.grid_type = t->grid_type;
 	if (t->grid_type == XSB_UNIONIZED &&
 	    (rc = build_material_index(d, t, !gen, (flags & XSB_BUILD_IDX32) != 0)))
+		return rc;
+	if (t->grid_type == XSB_HASH && (rc = build_hash_search(d, n_iso, n_gp, index_rows, pitch)))
 		return rc;
 	T.hash_du = (t->hash_bins > 0) ? 1.0 / t->hash_bins : 0.0;
 	pick_thresholds(T.thr);

@@ -15,6 +15,7 @@ constexpr uint64_t kLcgMask = 0x7FFFFFFFFFFFFFFFULL;  // mod 2^63
 constexpr int kMaterials = 12;
 constexpr int kPointDoubles = 6;  // energy + 5 XS channels (NuclideGridPoint)
 constexpr int kIvlDoubles = 12;   // interval record: upper point, then upper - lower
+constexpr int kHwinDoubles = 12;  // hash window record (Tables::hwin)
 
 enum GridType : int { kUnionized = 0, kNuclide = 1, kHash = 2 };
 
@@ -66,6 +67,16 @@ struct Tables {
 	int64_t midx_off[kMaterials];
 	int32_t midx_pitch[kMaterials];
 	int32_t midx_shift;
+	// hash grid, for lanes whose bins differ (the history rounds): each
+	// nuclide's energies on their own, row nuc at esoa + nuc * e_pitch
+	// (e_pitch >= n_gp + 16, a multiple of 8; +inf past n_gp), and per bin b
+	// and nuclide a 96-byte window record at hwin + (b * idx_pitch + nuc) *
+	// kHwinDoubles: {u_low, u_high} (int32 x 2, the two index grid entries
+	// the reference reads, Simulation.c:282-288), 8 bytes of padding, then
+	// E[u_low & ~1 ...] ten energies of the nuclide's row
+	const double *__restrict__ hwin;
+	const double *__restrict__ esoa;
+	int32_t e_pitch;
 };
 
 // ---------------------------------------------------------------------------
@@ -210,11 +221,65 @@ __device__ __forceinline__ int32_t ueg_bracket(const Tables &T, const uint8_t *r
 	                         : reinterpret_cast<const int32_t *>(row)[j];
 }
 
+// Hash grid, one lane's bracket of nuclide nuc in bin idx when the lanes'
+// bins differ (openmp-threading/Simulation.c:280-302).  One 96-byte record
+// holds u_low, u_high and the energies E[b .. b + 9], b = u_low rounded
+// down to even (six aligned 16-byte loads, one dependent step where the
+// index entries, then the bisection's probes, would take several); eight
+// more energies come from the nuclide's row where the bin spans further
+// (with n_gp / bins points per bin on average, a bin of the 45000-point grid
+// over 10000 bins exceeds ten in about 4% of cases).  With E[b .. u_high]
+// sorted, the count c of those <= p decides the reference's three cases:
+// p <= E[u_low] gives 0; c counting every one of them means p >= E[u_high],
+// giving n_gp - 1; else the bisection of [u_low, u_high] ends at b + c - 1.
+// A bin spanning more than eighteen energies, or unsorted rows, bisects.
+__device__ __forceinline__ int32_t hash_lower_window(const Tables &T, int nuc, int64_t idx, double p)
+{
+	const int32_t n_gp = T.n_gp;
+	const double2 *__restrict__ R =
+	    reinterpret_cast<const double2 *>(T.hwin + (idx * T.idx_pitch + nuc) * kHwinDoubles);
+	const double2 h = R[0];
+	const int32_t u_low = __double2loint(h.x), u_high = __double2hiint(h.x);
+	double v[10];
+#pragma unroll
+	for (int q = 0; q < 5; q++) {
+		const double2 x = R[1 + q];
+		v[2 * q] = x.x;
+		v[2 * q + 1] = x.y;
+	}
+	const int32_t b = u_low & ~1;
+	const int32_t top = u_high - b;  // position of u_high in the window
+	const double e_low = (u_low & 1) ? v[1] : v[0];
+	const double *__restrict__ E = T.esoa + (int64_t)nuc * T.e_pitch;
+	const bool sorted = (T.variant & kVariantSortedRows) != 0;
+	if (!sorted || top > 17) {
+		const double e_high = E[u_high];
+		return (p <= e_low)    ? 0
+		       : (p >= e_high) ? n_gp - 1
+		                       : grid_search_nuclide<1>(p, E, u_low, u_high);
+	}
+	int32_t c = 0;
+#pragma unroll
+	for (int m = 0; m < 10; m++)
+		c += (m <= top && v[m] <= p) ? 1 : 0;
+	if (top > 9) {
+		// in bounds: b + 17 <= n_gp + 15 < e_pitch
+		const double2 *__restrict__ w = reinterpret_cast<const double2 *>(E + b);
+#pragma unroll
+		for (int q = 5; q < 9; q++) {
+			const double2 x = w[q];
+			c += (2 * q <= top && x.x <= p) ? 1 : 0;
+			c += (2 * q + 1 <= top && x.y <= p) ? 1 : 0;
+		}
+	}
+	return !(e_low < p) ? 0 : (c == top + 1) ? n_gp - 1 : b + c - 1;
+}
+
 // mrow: the material's unionized index row (ueg_row), j: the nuclide's
 // position in the material list; both used on the unionized grid only.
-// COUNT (hash grid): lanes whose bins differ (the history rounds) count a
-// short range instead of bisecting it; where a wave's lanes mostly share a
-// bin the bisection's probes coalesce and are cheaper.
+// COUNT (hash grid): lanes whose bins differ (the history rounds) take
+// hash_lower_window; where a wave's lanes mostly share a bin the
+// bisection's probes coalesce and are cheaper.
 template <int GRID, int STRIDE, bool COUNT = false>
 __device__ __forceinline__ int32_t nuclide_lower(const Tables &T, const double *__restrict__ E_row,
                                                  int nuc, const uint8_t *__restrict__ mrow, int j,
@@ -222,7 +287,9 @@ __device__ __forceinline__ int32_t nuclide_lower(const Tables &T, const double *
 {
 	const int32_t n_gp = T.n_gp;
 	int32_t lower;
-	if (GRID == kUnionized) {
+	if (GRID == kHash && COUNT) {
+		lower = hash_lower_window(T, nuc, idx, p);
+	} else if (GRID == kUnionized) {
 		lower = ueg_bracket(T, mrow, j);
 	} else if (GRID == kNuclide) {
 		lower = grid_search_nuclide<STRIDE>(p, E_row, 0, n_gp - 1);
@@ -233,22 +300,7 @@ __device__ __forceinline__ int32_t nuclide_lower(const Tables &T, const double *
 		                           : T.index_grid[(idx + 1) * T.idx_pitch + nuc] + 1;
 		const double e_low = E_row[(int64_t)u_low * STRIDE];
 		const double e_high = E_row[(int64_t)u_high * STRIDE];
-		if (COUNT) {
-			// on sorted rows a range of at most kCount points is counted (its
-			// interior energies <= p, loaded together with the bounds) instead
-			// of bisected with a chain of dependent probes: the same index
-			constexpr int32_t kCount = 8;
-			int32_t cnt = 0;
-#pragma unroll
-			for (int32_t i = 1; i < kCount; i++) {
-				const double e = E_row[(int64_t)min(u_low + i, u_high) * STRIDE];  // in bounds
-				cnt += (u_low + i < u_high && e <= p) ? 1 : 0;
-			}
-			lower = (p <= e_low) ? 0 : (p >= e_high) ? n_gp - 1 : u_low + cnt;
-			if (!((T.variant & kVariantSortedRows) && u_high - u_low <= kCount) && e_low < p &&
-			    p < e_high)
-				lower = grid_search_nuclide<STRIDE>(p, E_row, u_low, u_high);
-		} else if (p <= e_low) {
+		if (p <= e_low) {
 			lower = 0;
 		} else if (p >= e_high) {
 			lower = n_gp - 1;
@@ -286,20 +338,31 @@ __device__ __forceinline__ int64_t lookup_index(const Tables &T, double p)
 }
 
 // FP64 interpolation of the 5 channels between points lo and hi
-// (openmp-threading/Simulation.c:306-322), accumulated into macro in the
-// reference's order (Simulation.c:372-373).
-__device__ __forceinline__ void accumulate_fp64(const double *__restrict__ pt, double p, double conc,
-                                                double macro[5])
+// (openmp-threading/Simulation.c:306-322) times the concentration: one
+// nuclide's terms of calculate_macro_xs (Simulation.c:372-373).
+__device__ __forceinline__ void term_fp64(const double *__restrict__ pt, double p, double conc,
+                                          double t[5])
 {
 	const double2 *q = reinterpret_cast<const double2 *>(pt);
 	const double2 a = q[0], b = q[1], c = q[2];  // lo: E t | el ab | fi nf
 	const double2 d = q[3], e = q[4], f = q[5];  // hi: E t | el ab | fi nf
 	const double fr = (d.x - p) / (d.x - a.x);
-	macro[0] += (d.y - fr * (d.y - a.y)) * conc;
-	macro[1] += (e.x - fr * (e.x - b.x)) * conc;
-	macro[2] += (e.y - fr * (e.y - b.y)) * conc;
-	macro[3] += (f.x - fr * (f.x - c.x)) * conc;
-	macro[4] += (f.y - fr * (f.y - c.y)) * conc;
+	t[0] = (d.y - fr * (d.y - a.y)) * conc;
+	t[1] = (e.x - fr * (e.x - b.x)) * conc;
+	t[2] = (e.y - fr * (e.y - b.y)) * conc;
+	t[3] = (f.x - fr * (f.x - c.x)) * conc;
+	t[4] = (f.y - fr * (f.y - c.y)) * conc;
+}
+
+// The terms accumulated into macro in the reference's order.
+__device__ __forceinline__ void accumulate_fp64(const double *__restrict__ pt, double p, double conc,
+                                                double macro[5])
+{
+	double t[5];
+	term_fp64(pt, p, conc, t);
+#pragma unroll
+	for (int k = 0; k < 5; k++)
+		macro[k] += t[k];
 }
 
 // accumulate_fp64 from an interval record (Tables::ivl): the differences

@@ -10,6 +10,7 @@
 // Particle q's state between rounds: seed[q], the LCG state from which its
 // next energy and material are drawn, and psum[q], its verification sum.
 // The whole chain of rounds is captured once in a hipGraph and replayed.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -31,11 +32,16 @@ __global__ __launch_bounds__(kHBlock) void k_hinit(Tables T, int64_t P, int look
 	}
 }
 
-// key = material << 16 | top 16 bits of the next energy; value = particle
+// key = material << 16 | top 16 bits of the next energy; value = particle.
+// Also empties the round's re-evaluation queue (the previous round's
+// k_hfallback has finished with it).
 __global__ __launch_bounds__(kSortBlock) void k_hkeys(Tables T, int64_t P,
                                                       const uint64_t *__restrict__ seed,
-                                                      uint2 *pairs, uint32_t *hist, int ntiles)
+                                                      uint2 *pairs, uint32_t *hist, int ntiles,
+                                                      uint32_t *queue)
 {
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		queue[0] = 0;
 	__shared__ uint32_t h[kBins];
 	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
 		h[d] = 0;
@@ -63,13 +69,20 @@ __global__ __launch_bounds__(kSortBlock) void k_hkeys(Tables T, int64_t P,
 // energy and material, look up, add argmax + 1, fast-forward the state by
 // the number of channels above 1.0.  Lanes past P repeat the last particle
 // and write nothing.  ids: the sorted {key, particle} pairs.
+// Mixed precision: a lookup the bounds cannot decide is not re-evaluated
+// here -- its lane's FP64 loop would hold the wave (most often a fuel wave,
+// the round's longest, as the undecided lookups are mostly fuel lookups
+// with many terms) for another full chain of dependent loads -- but queued
+// (queue[0] counts, queue[1 ..] particles) for k_hfallback, its particle's
+// state left as it was.
 template <int GRID, int PREC, int CHUNK>
 __global__ __launch_bounds__(kHBlock) void k_hround(Tables T, int64_t P,
                                                     const uint2 *__restrict__ ids,
                                                     uint64_t *__restrict__ seed,
                                                     uint32_t *__restrict__ psum,
                                                     unsigned long long *vsum,
-                                                    unsigned long long *nfb)
+                                                    unsigned long long *nfb,
+                                                    uint32_t *__restrict__ queue)
 {
 	const int64_t i = (int64_t)xcd_block(kHXcdRun) * kHBlock + threadIdx.x;
 	const bool valid = i < P;
@@ -80,21 +93,99 @@ __global__ __launch_bounds__(kHBlock) void k_hround(Tables T, int64_t P,
 	int64_t idx;
 	double macro[5];
 	uint32_t fb = 0;
-	const int am = sorted_lookup<GRID, PREC, CHUNK, true>(T, p, mat, T.variant & kVariantWave, idx,
-	                                                      macro, fb, nullptr);
+	const int am = sorted_lookup<GRID, PREC, CHUNK, true, true, PREC != XSB_XS_FP64>(
+	    T, p, mat, T.variant & kVariantWave, idx, macro, fb, nullptr);
+	const bool done = valid && !fb;
+	if (valid && fb)
+		queue[1 + atomicAdd(queue, 1u)] = q;
 	uint32_t n_forward = 0;
 #pragma unroll
 	for (int c = 0; c < 5; c++)
 		n_forward += (macro[c] > 1.0) ? 1u : 0u;
 	if (n_forward)
 		s = ff_jump(s, n_forward, T.ff_tab);
-	if (valid) {
+	if (done) {
 		seed[q] = s;
 		psum[q] += (uint32_t)am + 1u;
 	}
-	block_add_u64<kHBlock>(valid ? (uint32_t)am + 1u : 0u, vsum);
+	block_add_u64<kHBlock>(done ? (uint32_t)am + 1u : 0u, vsum);
 	if (PREC != XSB_XS_FP64)
 		block_add_u64<kHBlock>(valid ? fb : 0u, nfb);
+}
+
+// The queued lookups of a round in FP64, one workgroup each: thread t
+// computes the terms xs_c * conc of nuclides t, t + kFbBlock, ...
+// (Simulation.c:360-374) into LDS, threads 0 .. 4 add channel c's terms in
+// list order -- the sums of macro_xs_fp64 bit for bit, after a dependent
+// chain of one or two nuclides instead of the whole list -- and thread 0
+// completes the round for the particle as k_hround does.  Nuclides go
+// through LDS kFbChunk at a time.
+constexpr int kFbBlock = 256;
+constexpr int kFbChunk = 512;
+
+template <int GRID>
+__global__ __launch_bounds__(kFbBlock) void k_hfallback(Tables T, const uint32_t *__restrict__ queue,
+                                                        uint64_t *__restrict__ seed,
+                                                        uint32_t *__restrict__ psum,
+                                                        unsigned long long *vsum)
+{
+	__shared__ double t[kFbChunk * 5];
+	__shared__ double sums[5];
+	const int tid = threadIdx.x;
+	const uint32_t n = queue[0];
+	for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+		const uint32_t q = queue[1 + w];
+		uint64_t s = seed[q];
+		const double p = lcg_double(s);
+		const int mat = pick_mat(s, T.thr);
+		const int64_t idx = lookup_index<GRID>(T, p);
+		const int nn = T.num_nucs[mat];
+		const int32_t *__restrict__ ml = T.mats + mat * T.max_num_nucs;
+		const double *__restrict__ cl = T.concs + mat * T.max_num_nucs;
+		const uint8_t *__restrict__ mrow = (GRID == kUnionized) ? ueg_row(T, mat, idx) : nullptr;
+		double acc = 0.0;  // threads 0 .. 4: channel `tid`
+		for (int j0 = 0; j0 < nn; j0 += kFbChunk) {
+			const int nc = min(kFbChunk, nn - j0);
+#pragma unroll 2
+			for (int jj = tid; jj < nc; jj += kFbBlock) {
+				const int j = j0 + jj;
+				const int nuc = ml[j];
+				const double *row = T.grid + (int64_t)nuc * T.n_gp * kPointDoubles;
+				const int32_t lower = nuclide_lower<GRID, kPointDoubles, true>(T, row, nuc, mrow, j,
+				                                                               idx, p);
+				double tt[5];
+				term_fp64(row + (int64_t)lower * kPointDoubles, p, cl[j], tt);
+#pragma unroll
+				for (int c = 0; c < 5; c++)
+					t[jj * 5 + c] = tt[c];
+			}
+			__syncthreads();
+			if (tid < 5)
+				for (int jj = 0; jj < nc; jj++)
+					acc += t[jj * 5 + tid];
+			__syncthreads();
+		}
+		if (tid < 5)
+			sums[tid] = acc;
+		__syncthreads();
+		if (tid == 0) {
+			double macro[5];
+#pragma unroll
+			for (int c = 0; c < 5; c++)
+				macro[c] = sums[c];
+			const int am = argmax5(macro);
+			uint32_t n_forward = 0;
+#pragma unroll
+			for (int c = 0; c < 5; c++)
+				n_forward += (macro[c] > 1.0) ? 1u : 0u;
+			if (n_forward)
+				s = ff_jump(s, n_forward, T.ff_tab);
+			seed[q] = s;
+			psum[q] += (uint32_t)am + 1u;
+			atomicAdd(vsum, (unsigned long long)(am + 1));
+		}
+		__syncthreads();
+	}
 }
 
 // Per-particle outputs: the sum widened to 64 bits, and the final state
@@ -122,30 +213,43 @@ __global__ void k_hreset(unsigned long long *acc)
 		acc[threadIdx.x] = 0;
 }
 
+// k_hfallback's grid: the queue holds a few hundred lookups per million
+// with FP32 tables, about a tenth of them with BF16's.
+static unsigned fallback_blocks(int prec, int64_t P)
+{
+	const int64_t want = prec == XSB_XS_BF16 ? 4096 : 512;
+	return (unsigned)std::max<int64_t>(1, std::min(want, P));
+}
+
 template <int GRID, int PREC>
 static void launch_round(Device *d, hipStream_t s, int64_t P, const uint2 *ids, uint64_t *seed,
-                         uint32_t *psum, unsigned long long *acc)
+                         uint32_t *psum, unsigned long long *acc, uint32_t *queue)
 {
 	const unsigned nb = (unsigned)((P + kHBlock - 1) / kHBlock);
 	constexpr int CHUNK = kIdxChunk;
 	hipLaunchKernelGGL((k_hround<GRID, PREC, CHUNK>), dim3(nb), dim3(kHBlock), 0, s, d->T, P, ids,
-	                   seed, psum, acc, acc + 1);
+	                   seed, psum, acc, acc + 1, queue);
+	if (PREC != XSB_XS_FP64)
+		hipLaunchKernelGGL((k_hfallback<GRID>), dim3(fallback_blocks(PREC, P)), dim3(kFbBlock), 0, s,
+		                   d->T, queue, seed, psum, acc);
 }
 
 template <int PREC>
 static void launch_round_grid(Device *d, hipStream_t s, int64_t P, const uint2 *ids,
-                              uint64_t *seed, uint32_t *psum, unsigned long long *acc)
+                              uint64_t *seed, uint32_t *psum, unsigned long long *acc,
+                              uint32_t *queue)
 {
 	switch (d->T.grid_type) {
-	case kUnionized: launch_round<kUnionized, PREC>(d, s, P, ids, seed, psum, acc); break;
-	case kNuclide: launch_round<kNuclide, PREC>(d, s, P, ids, seed, psum, acc); break;
-	default: launch_round<kHash, PREC>(d, s, P, ids, seed, psum, acc); break;
+	case kUnionized: launch_round<kUnionized, PREC>(d, s, P, ids, seed, psum, acc, queue); break;
+	case kNuclide: launch_round<kNuclide, PREC>(d, s, P, ids, seed, psum, acc, queue); break;
+	default: launch_round<kHash, PREC>(d, s, P, ids, seed, psum, acc, queue); break;
 	}
 }
 
 struct HistoryBuffers {
 	uint64_t *seed;
 	uint32_t *psum, *hist, *part;
+	uint32_t *queue;  // [1 + P]: count, then the particles to re-evaluate in FP64
 	uint2 *pA, *pB;  // {key, particle} pairs, two buffers
 	uint64_t *out_sum, *out_seed;
 	unsigned long long *acc;
@@ -164,17 +268,17 @@ static int enqueue_rounds(Device *d, hipStream_t s, int64_t P, int lookups, uint
 	for (int r = 0; r < lookups; r++) {
 		uint2 *pA = B.pA, *pB = B.pB;
 		hipLaunchKernelGGL(k_hkeys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, P, B.seed, pA,
-		                   B.hist, ntiles);
+		                   B.hist, ntiles, B.queue);
 		XSB_HIP(hipGetLastError());
 		int rc = order_pairs(s, P, 2, true, &pA, &pB, B.hist, B.part);
 		if (rc)
 			return rc;
 		if (prec == XSB_XS_FP64)
-			launch_round_grid<XSB_XS_FP64>(d, s, P, pA, B.seed, B.psum, B.acc);
+			launch_round_grid<XSB_XS_FP64>(d, s, P, pA, B.seed, B.psum, B.acc, B.queue);
 		else if (prec == XSB_XS_FP32)
-			launch_round_grid<XSB_XS_FP32>(d, s, P, pA, B.seed, B.psum, B.acc);
+			launch_round_grid<XSB_XS_FP32>(d, s, P, pA, B.seed, B.psum, B.acc, B.queue);
 		else
-			launch_round_grid<XSB_XS_BF16>(d, s, P, pA, B.seed, B.psum, B.acc);
+			launch_round_grid<XSB_XS_BF16>(d, s, P, pA, B.seed, B.psum, B.acc, B.queue);
 		XSB_HIP(hipGetLastError());
 	}
 	if (B.out_sum || B.out_seed) {
@@ -191,9 +295,10 @@ int run_history_rounds(Device *d, const xsb_history_args *a, xsb_result *r)
 	const int L = a->lookups_per_particle;
 	hipStream_t s = d->stream;
 	const bool outs = a->out_particle_sum || a->out_final_seed;
-	// the eight buffers below, each rounded up to 256 bytes by take()
+	// the nine buffers below, each rounded up to 256 bytes by take()
 	const size_t bytes = (size_t)P * (8 + 4 + 16) + (size_t)sort_hist_len(P) * 4 +
-	                     (size_t)sort_parts(P) * 4 + (outs ? (size_t)P * 16 : 0) + 8 * 256;
+	                     (size_t)sort_parts(P) * 4 + (outs ? (size_t)P * 16 : 0) +
+	                     ((size_t)P + 1) * 4 + 9 * 256;
 	void *buf = nullptr;
 	int rc = scratch_reserve(d, 3, bytes, &buf);
 	if (rc)
@@ -213,6 +318,7 @@ int run_history_rounds(Device *d, const xsb_history_args *a, xsb_result *r)
 	B.part = (uint32_t *)take((size_t)sort_parts(P) * 4);
 	B.out_sum = a->out_particle_sum ? (uint64_t *)take((size_t)P * 8) : nullptr;
 	B.out_seed = a->out_final_seed ? (uint64_t *)take((size_t)P * 8) : nullptr;
+	B.queue = (uint32_t *)take(((size_t)P + 1) * 4);
 	B.acc = d->d_acc + 8;
 
 	// the graph is keyed on everything its nodes hold

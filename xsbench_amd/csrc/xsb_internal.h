@@ -47,6 +47,8 @@ struct Device {
 	// owned allocations
 	double *d_ueg = nullptr;
 	int32_t *d_index = nullptr;
+	double *d_hwin = nullptr;   // hash grid: window records per bin and nuclide (Tables::hwin)
+	double *d_esoa = nullptr;   // hash grid: energies by nuclide (Tables::esoa)
 	uint8_t *d_midx = nullptr;  // per-material unionized index tables (uint16 or int32)
 	double *d_grid = nullptr;
 	double *d_ivl = nullptr;    // FP64 interval records (Tables::ivl)

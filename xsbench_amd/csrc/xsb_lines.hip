@@ -43,6 +43,8 @@ enum LineRegion : int {
 	kRegGrid,     // FP64 points
 	kRegXs32,     // FP32 records
 	kRegXs16,     // BF16 records
+	kRegHwin,     // hash window records
+	kRegEsoa,     // hash energy rows
 	kRegCount
 };
 
@@ -87,8 +89,11 @@ __device__ __forceinline__ int64_t mark_ueg_search(const Tables &T, const LineMa
 }
 
 // The reads of one lookup (p, mat) on precision PREC's tables.  WAVE: the
-// sorted kernels' interval records (unionized FP64); else points.
-template <int GRID, int PREC, bool WAVE>
+// sorted kernels' interval records (unionized FP64); else points.  HIST:
+// the history rounds' hash bracket (hash_lower_window: the bin's window
+// record, more of the energy row where the bin spans further); else the
+// index grid and the bin's range of points.
+template <int GRID, int PREC, bool WAVE, bool HIST = false>
 __device__ __forceinline__ void mark_lookup(const Tables &T, const LineMap &L, double p, int mat)
 {
 	const int nn = T.num_nucs[mat];
@@ -123,6 +128,32 @@ __device__ __forceinline__ void mark_lookup(const Tables &T, const LineMap &L, d
 		const bool last = idx == T.hash_bins - 1;
 		for (int j = 0; j < nn; j++) {
 			const int nuc = ml[j];
+			if (HIST) {
+				const double *R = T.hwin + (idx * T.idx_pitch + nuc) * kHwinDoubles;
+				touch(L, kRegHwin, R, kHwinDoubles * 8);
+				const int32_t u_low = __double2loint(R[0]), u_high = __double2hiint(R[0]);
+				const int32_t b = u_low & ~1, top = u_high - b;
+				const double *E = T.esoa + (int64_t)nuc * T.e_pitch;
+				if (!(T.variant & kVariantSortedRows) || top > 17) {
+					touch(L, kRegEsoa, E + u_high, 8);
+					if (E[u_low] < p && p < E[u_high]) {
+						int32_t lo = u_low, hi = u_high;
+						while (hi - lo > 1) {
+							const int32_t mid = lo + ((hi - lo) >> 1);
+							touch(L, kRegEsoa, E + mid, 8);
+							if (E[mid] > p)
+								hi = mid;
+							else
+								lo = mid;
+						}
+					}
+				} else if (top > 9) {
+					touch(L, kRegEsoa, E + b + 10, 8 * 8);
+				}
+				const int32_t lower = nuclide_lower<kHash, REC, true>(T, nullptr, nuc, nullptr, j, idx, p);
+				touch(L, REG, base + ((int64_t)nuc * n_gp + lower) * REC, 2 * REC * 8);
+				continue;
+			}
 			const int32_t *w = T.index_grid + idx * T.idx_pitch + nuc;
 			touch(L, kRegIndex, w, 4);
 			if (!last)
@@ -195,7 +226,7 @@ __global__ __launch_bounds__(256) void k_lines_history(Tables T, LineMap L, int6
 	LineMap Lr = L;
 	for (int r = 0; r < lookups; r++) {
 		Lr.bits = L.bits + (int64_t)r * set_words;
-		mark_lookup<GRID, PREC, true>(T, Lr, p, mat);
+		mark_lookup<GRID, PREC, true, true>(T, Lr, p, mat);
 		const int64_t idx = lookup_index<GRID>(T, p);
 		double macro[5];
 		macro_xs_fp64<GRID>(T, p, mat, idx, macro);
@@ -236,8 +267,8 @@ static int64_t line_regions(const Device *d, LineMap *L)
 	const Tables &T = d->T;
 	const int64_t n = (int64_t)T.n_iso * T.n_gp;
 	const int64_t idx_rows = T.grid_type == kUnionized ? 0 : (T.grid_type == kHash ? T.hash_bins : 0);
-	const void *base[kRegCount] = {T.ueg,  T.ueg_acc, T.midx, T.ivl,  T.ivl32,
-	                               T.index_grid, T.grid, T.xs32, T.xs16};
+	const void *base[kRegCount] = {T.ueg,  T.ueg_acc, T.midx, T.ivl,  T.ivl32, T.index_grid,
+	                               T.grid, T.xs32, T.xs16, T.hwin, T.esoa};
 	const int64_t bytes[kRegCount] = {
 	    T.ueg ? T.n_union * 8 : 0,
 	    T.ueg_acc ? ((int64_t)(1 << kUegAccBits) + 1) * 4 : 0,
@@ -247,7 +278,9 @@ static int64_t line_regions(const Device *d, LineMap *L)
 	    T.index_grid ? idx_rows * T.idx_pitch * 4 : 0,
 	    n * kPointDoubles * 8,
 	    T.xs32 ? n * kRec32Doubles * 8 : 0,
-	    T.xs16 ? n * kRec16Doubles * 8 : 0};
+	    T.xs16 ? n * kRec16Doubles * 8 : 0,
+	    T.hwin ? idx_rows * T.idx_pitch * kHwinDoubles * 8 : 0,
+	    T.esoa ? (int64_t)T.n_iso * T.e_pitch * 8 : 0};
 	int64_t words = 0;
 	for (int r = 0; r < kRegCount; r++) {
 		L->base[r] = (const uint8_t *)base[r];

@@ -97,6 +97,27 @@ __device__ __forceinline__ void load_xs(const double *rec, float x[5])
 	}
 }
 
+// FP32 records lo and lo + 1 (32-byte aligned, adjacent: {E, xs[5], pad}
+// each) as four 16-byte loads.
+__device__ __forceinline__ void load_pair32(const double *lo, double &e_lo, float xl[5],
+                                            double &e_hi, float xh[5])
+{
+	const uint4 *q = reinterpret_cast<const uint4 *>(lo);
+	const uint4 a = q[0], b = q[1], c = q[2], d = q[3];
+	e_lo = __hiloint2double((int)a.y, (int)a.x);
+	xl[0] = __uint_as_float(a.z);
+	xl[1] = __uint_as_float(a.w);
+	xl[2] = __uint_as_float(b.x);
+	xl[3] = __uint_as_float(b.y);
+	xl[4] = __uint_as_float(b.z);
+	e_hi = __hiloint2double((int)c.y, (int)c.x);
+	xh[0] = __uint_as_float(c.z);
+	xh[1] = __uint_as_float(c.w);
+	xh[2] = __uint_as_float(d.x);
+	xh[3] = __uint_as_float(d.y);
+	xh[4] = __uint_as_float(d.z);
+}
+
 // The two FP32 weights of a nuclide's term from the FP64 fraction f:
 // fr = RN(f), gr = RN(1 - f); wmin keeps the smallest weight of the lookup
 // (every term is a non-negative combination when it stays >= 0).
@@ -204,8 +225,9 @@ __device__ __forceinline__ void mixed_term_ivl2(const double *__restrict__ Q, bo
 // concentration.  fp64(m) writes this lane's FP64 sums.  WAVE: fp64 is a
 // wave-cooperative path that every lane of the wave must enter, so it runs
 // for the whole wave as soon as one lane needs it (cheaper than one lane's
-// FP64 loop with its wave idle).
-template <int PREC, bool HISTORY, bool WAVE, class F64>
+// FP64 loop with its wave idle).  DEFER: fp64 is not called; the caller
+// re-evaluates the lookups with *fell_back = 1 itself.
+template <int PREC, bool HISTORY, bool WAVE, bool DEFER = false, class F64>
 __device__ __forceinline__ int mixed_finish(const double s[5], float A, float wmin, bool nonneg,
                                             double macro[5], uint32_t *fell_back, double *bound,
                                             F64 &&fp64)
@@ -241,7 +263,7 @@ __device__ __forceinline__ int mixed_finish(const double s[5], float A, float wm
 	for (int k = 0; k < 5; k++)
 		macro[k] = s[k];
 	*fell_back = ambiguous ? 1u : 0u;
-	if (WAVE ? __ballot(ambiguous) == 0ULL : !ambiguous)
+	if (DEFER || (WAVE ? __ballot(ambiguous) == 0ULL : !ambiguous))
 		return am;
 	double m64[5];
 	fp64(m64);
@@ -262,7 +284,7 @@ __device__ __forceinline__ const float *mixed_consts(const Tables &T, int mat)
 }
 
 // calculate_macro_xs on the mixed tables, one lane per lookup.
-template <int GRID, int PREC, bool HISTORY = false>
+template <int GRID, int PREC, bool HISTORY = false, bool DEFER = false>
 __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat, int64_t idx,
                                               double macro[5], uint32_t *fell_back,
                                               double *bound = nullptr)
@@ -285,8 +307,14 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 			return;
 		}
 		const double *lo = row + (int64_t)lower * REC;
-		const double *hi = lo + REC;
 		float xl[5], xh[5];
+		if (PREC == XSB_XS_FP32) {
+			double e_lo, e_hi;
+			load_pair32(lo, e_lo, xl, e_hi, xh);
+			mixed_term(e_lo, xl, e_hi, xh, p, ceff[j], cbnd[j], sum, A, wmin);
+			return;
+		}
+		const double *hi = lo + REC;
 		load_xs<PREC>(lo, xl);
 		load_xs<PREC>(hi, xh);
 		mixed_term(lo[0], xl, hi[0], xh, p, ceff[j], cbnd[j], sum, A, wmin);
@@ -300,10 +328,10 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 		for (int j = 0; j < nn; j++)
 			step(j);
 	}
-	return mixed_finish<PREC, HISTORY, false>(sum, A, wmin, (T.variant & kVariantNonNeg) != 0, macro,
-	                                          fell_back, bound, [&](double m64[5]) {
-		                                          macro_xs_fp64<GRID, HISTORY>(T, p, mat, idx, m64);
-	                                          });
+	return mixed_finish<PREC, HISTORY, false, DEFER>(sum, A, wmin, (T.variant & kVariantNonNeg) != 0,
+	                                                 macro, fell_back, bound, [&](double m64[5]) {
+		                                                 macro_xs_fp64<GRID, HISTORY>(T, p, mat, idx, m64);
+	                                                 });
 }
 
 // One nuclide of the wave-uniform path on the mixed tables, bracket k per
@@ -375,7 +403,7 @@ __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc,
 
 // calculate_macro_xs, unionized grid, mixed tables, for a wave whose lanes
 // all hold material m; same terms and decision as macro_xs_mixed.
-template <int PREC, int CHUNK, bool HISTORY, bool SHARE3 = true>
+template <int PREC, int CHUNK, bool HISTORY, bool SHARE3 = true, bool DEFER = false>
 __device__ __forceinline__ int macro_ueg_wave_mixed(const Tables &T, double p, int m, int64_t idx,
                                                     double macro[5], uint32_t *fell_back,
                                                     double *bound)
@@ -387,15 +415,15 @@ __device__ __forceinline__ int macro_ueg_wave_mixed(const Tables &T, double p, i
 	ueg_wave_rows<CHUNK>(T, m, idx, [&](int j, int nuc, int k) {
 		ueg_wave_nuclide_mixed<PREC, SHARE3, true>(T, nuc, k, p, ceff[j], cbnd[j], sum, A, wmin);
 	});
-	return mixed_finish<PREC, HISTORY, true>(sum, A, wmin, (T.variant & kVariantNonNeg) != 0, macro,
-	                                         fell_back, bound, [&](double m64[5]) {
-		                                         macro_ueg_wave<CHUNK, SHARE3>(T, p, m, idx, m64);
-	                                         });
+	return mixed_finish<PREC, HISTORY, true, DEFER>(sum, A, wmin, (T.variant & kVariantNonNeg) != 0,
+	                                                macro, fell_back, bound, [&](double m64[5]) {
+		                                                macro_ueg_wave<CHUNK, SHARE3>(T, p, m, idx, m64);
+	                                                });
 }
 
 // calculate_macro_xs, hash grid, mixed tables, for a wave whose lanes share
 // the material and the hash bin; same terms and decision as macro_xs_mixed.
-template <int PREC, bool HISTORY, bool SHARE3 = true>
+template <int PREC, bool HISTORY, bool SHARE3 = true, bool DEFER = false>
 __device__ __forceinline__ int macro_hash_wave_mixed(const Tables &T, double p, int m, int64_t idx,
                                                      double macro[5], uint32_t *fell_back,
                                                      double *bound)
@@ -408,10 +436,10 @@ __device__ __forceinline__ int macro_hash_wave_mixed(const Tables &T, double p, 
 	hash_wave_rows<REC>(T, mixed_row<PREC>(T, 0), p, m, idx, [&](int j, int nuc, int k) {
 		ueg_wave_nuclide_mixed<PREC, SHARE3>(T, nuc, k, p, ceff[j], cbnd[j], sum, A, wmin);
 	});
-	return mixed_finish<PREC, HISTORY, true>(sum, A, wmin, (T.variant & kVariantNonNeg) != 0, macro,
-	                                         fell_back, bound, [&](double m64[5]) {
-		                                         macro_hash_wave<SHARE3>(T, p, m, idx, m64);
-	                                         });
+	return mixed_finish<PREC, HISTORY, true, DEFER>(sum, A, wmin, (T.variant & kVariantNonNeg) != 0,
+	                                                macro, fell_back, bound, [&](double m64[5]) {
+		                                                macro_hash_wave<SHARE3>(T, p, m, idx, m64);
+	                                                });
 }
 
 }  // namespace xsb

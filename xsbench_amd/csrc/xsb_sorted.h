@@ -75,8 +75,11 @@ __device__ __forceinline__ void write_tile_hist(const uint32_t *h, uint32_t *his
 // loop.  SHARE3 selects the wave paths' two-interval step (ueg_wave_nuclide).  Returns the argmax channel; idx gets the grid-level index,
 // macro the sums, fb the fallback flag (mixed precision).
 // HISTORY: the caller also tests macro_c > 1.0, so mixed precision falls
-// back to FP64 near 1.0 too and then returns the FP64 sums.
-template <int GRID, int PREC, int CHUNK, bool HISTORY = false, bool SHARE3 = true>
+// back to FP64 near 1.0 too and then returns the FP64 sums.  DEFER: a mixed
+// lookup that would fall back returns fb = 1 without its FP64 sums; the
+// caller re-evaluates it.
+template <int GRID, int PREC, int CHUNK, bool HISTORY = false, bool SHARE3 = true,
+          bool DEFER = false>
 __device__ __forceinline__ int sorted_lookup(const Tables &T, double p, int mat, bool wave,
                                              int64_t &idx, double macro[5], uint32_t &fb,
                                              double *bnd)
@@ -98,19 +101,20 @@ __device__ __forceinline__ int sorted_lookup(const Tables &T, double p, int mat,
 				macro_hash_wave<SHARE3>(T, p, m0, i0, macro);
 				return argmax5(macro);
 			}
-			return macro_hash_wave_mixed<PREC, HISTORY, SHARE3>(T, p, m0, i0, macro, &fb, bnd);
+			return macro_hash_wave_mixed<PREC, HISTORY, SHARE3, DEFER>(T, p, m0, i0, macro, &fb,
+			                                                           bnd);
 		}
 	} else if (GRID == kUnionized && PREC != XSB_XS_FP64) {
 		const int m0 = __builtin_amdgcn_readfirstlane(mat);
 		if (wave && __ballot(mat != m0) == 0ULL)
-			return macro_ueg_wave_mixed<PREC, CHUNK, HISTORY, SHARE3>(T, p, m0, idx, macro, &fb,
-			                                                          bnd);
+			return macro_ueg_wave_mixed<PREC, CHUNK, HISTORY, SHARE3, DEFER>(T, p, m0, idx, macro,
+			                                                                 &fb, bnd);
 	}
 	if (PREC == XSB_XS_FP64) {
 		macro_xs_fp64<GRID, HISTORY>(T, p, mat, idx, macro);
 		return argmax5(macro);
 	}
-	return macro_xs_mixed<GRID, PREC, HISTORY>(T, p, mat, idx, macro, &fb, bnd);
+	return macro_xs_mixed<GRID, PREC, HISTORY, DEFER>(T, p, mat, idx, macro, &fb, bnd);
 }
 
 }  // namespace xsb
