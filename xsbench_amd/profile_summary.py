@@ -7,7 +7,10 @@ summaries kept with the repository.
 roofline: DIR holds one profile_driver.py workload -- its meta.json (from the
 --kernel-trace run) and the counter passes rd/ (TCC_EA0_RDREQ, _32B,
 TCC_BUBBLE), wr/ (TCC_EA0_WRREQ, _64B, TCC_HIT, TCC_MISS), dram/
-(TCC_EA0_RDREQ_DRAM, TCC_READ_SECTORS, TCC_REQ, TCC_READ) and optionally sq/.
+(TCC_EA0_RDREQ_DRAM, TCC_READ_SECTORS, TCC_REQ, TCC_READ) and optionally sq/
+and tcp/ (TCP_TOTAL_CACHE_ACCESSES, TA_FLAT_READ_WAVEFRONTS, TA_TA_BUSY,
+GRBM_GUI_ACTIVE: 64-byte L1 accesses per vector load instruction and the
+share of cycles the 256 texture-address units were busy).
 HBM bytes per launch count every request to the memory side by its size:
 reads 32 B (RDREQ_32B), 128 B (TCC_BUBBLE) or 64 B (the rest), scaled by the
 factor that makes the same formula give the plain copy kernel's known bytes;
@@ -119,6 +122,13 @@ def roofline(args):
         out.append(f"dram_share_of_read_requests = "
                    f"{dr['TCC_EA0_RDREQ_DRAM_sum'] / max(rd['TCC_EA0_RDREQ_sum'], 1):.4f}")
         out.append(f"l2_read_sectors_bytes = {32.0 * dr['TCC_READ_SECTORS_sum']:.6e}")
+    tcp_dir = os.path.join(d, "tcp")
+    if os.path.isdir(tcp_dir):
+        tc, _ = counters(tcp_dir, k)
+        xcd_cycles = tc["GRBM_GUI_ACTIVE"] / 8.0  # the counter sums the 8 XCDs
+        out.append(f"l1_accesses_per_vmem_instruction = "
+                   f"{tc['TCP_TOTAL_CACHE_ACCESSES_sum'] / max(tc['TA_FLAT_READ_WAVEFRONTS_sum'], 1):.1f}")
+        out.append(f"ta_busy = {tc['TA_TA_BUSY_sum'] / (256.0 * max(xcd_cycles, 1)):.3f}")
     sq_dir = os.path.join(d, "sq")
     if os.path.isdir(sq_dir):
         sq, _ = counters(sq_dir, k)
