@@ -91,6 +91,19 @@ def test_window_matches_oracle(orc, name, k):
     assert same_bits(out["macro"], macro) and np.array_equal(out["argmax"], am)
 
 
+def test_sorted_schedule_matches_reference_order_all_17m_lookups():
+    """every lookup of the large unionized workload, macro XS bit for bit and
+    argmax: the sorted schedule (wave-shared interval records, per-material
+    index rows) against the reference-order kernel (points, the index grid's
+    brackets)"""
+    inp, sd, dev = data("large_unionized")
+    n = 17_000_000
+    _, ref = dev.event(n, 0, kernel_id=X.K_BASELINE, detail=True)
+    _, got = dev.event(n, 0, kernel_id=X.K_MAT_E_SORT, detail=True)
+    assert same_bits(got["macro"], ref["macro"])
+    assert np.array_equal(got["argmax"], ref["argmax"])
+
+
 # ---------------------------------------------------------------------------
 # full-size hashes
 # ---------------------------------------------------------------------------
