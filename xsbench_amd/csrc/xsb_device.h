@@ -365,6 +365,22 @@ __device__ __forceinline__ void accumulate_fp64(const double *__restrict__ pt, d
 		macro[k] += t[k];
 }
 
+// The tables as the constant address space: a wave-uniform load through
+// such a pointer becomes a scalar-cache load even in kernels that also
+// write global memory (the persistent batched kernel's atomics), where the
+// compiler cannot otherwise show that a table load is not clobbered.  The
+// tables are read-only while any kernel runs.
+typedef const __attribute__((address_space(4))) double *const_table_ptr;
+typedef const __attribute__((address_space(4))) int32_t *const_index_ptr;
+__device__ __forceinline__ const_table_ptr as_table(const double *p)
+{
+	return (const_table_ptr)p;
+}
+__device__ __forceinline__ const_index_ptr as_table(const int32_t *p)
+{
+	return (const_index_ptr)p;
+}
+
 // accumulate_fp64 from an interval record (Tables::ivl): the differences
 // hi - lo were rounded once when the record was built, as the reference
 // rounds them per lookup, so the result is the same.  Every operand but p
@@ -372,15 +388,13 @@ __device__ __forceinline__ void accumulate_fp64(const double *__restrict__ pt, d
 __device__ __forceinline__ void accumulate_ivl(const double *__restrict__ R, double p, double conc,
                                                double macro[5])
 {
-	const double2 *q = reinterpret_cast<const double2 *>(R);
-	const double2 a = q[0], b = q[1], c = q[2];  // hi: E t | el ab | fi nf
-	const double2 d = q[3], e = q[4], f = q[5];  // hi - lo: E t | el ab | fi nf
-	const double fr = (a.x - p) / d.x;
-	macro[0] += (a.y - fr * d.y) * conc;
-	macro[1] += (b.x - fr * e.x) * conc;
-	macro[2] += (b.y - fr * e.y) * conc;
-	macro[3] += (c.x - fr * f.x) * conc;
-	macro[4] += (c.y - fr * f.y) * conc;
+	const const_table_ptr r = as_table(R);  // hi: E t el ab fi nf, then hi - lo
+	const double fr = (r[0] - p) / r[6];
+	macro[0] += (r[1] - fr * r[7]) * conc;
+	macro[1] += (r[2] - fr * r[8]) * conc;
+	macro[2] += (r[3] - fr * r[9]) * conc;
+	macro[3] += (r[4] - fr * r[10]) * conc;
+	macro[4] += (r[5] - fr * r[11]) * conc;
 }
 
 // calculate_macro_xs (openmp-threading/Simulation.c:326-375), FP64.
@@ -480,7 +494,10 @@ __device__ __forceinline__ void ueg_wave_nuclide(const Tables &T, int nuc, int k
 	if (IVL) {
 		const double *R = T.ivl + i0 * kIvlDoubles;
 		if (__ballot(d != 0u) == 0ULL) {
+			// the barrier keeps this call from being merged with the per-lane
+			// one below into one call on a lane-varying pointer (vector loads)
 			accumulate_ivl(R, p, conc, macro);
+			__builtin_amdgcn_sched_barrier(0);
 		} else if (SHARE3 && __ballot(d > 1u) == 0ULL) {
 			accumulate_ivl_select(R, d, p, conc, macro);
 			__builtin_amdgcn_sched_barrier(0);
@@ -525,8 +542,8 @@ template <int CHUNK, class F>
 __device__ __forceinline__ void ueg_wave_rows(const Tables &T, int m, int64_t idx, F &&f)
 {
 	static_assert(CHUNK % 8 == 0, "staging depth is whole 16-byte vectors of either width");
-	const int nn = T.num_nucs[m];
-	const int32_t *__restrict__ ml = T.mats + m * T.max_num_nucs;
+	const int nn = as_table(T.num_nucs)[m];
+	const const_index_ptr ml = as_table(T.mats) + m * T.max_num_nucs;
 	const uint8_t *__restrict__ row = ueg_row(T, m, idx);
 	const bool narrow = T.midx_shift == 1;
 	int j0 = 0;
@@ -587,7 +604,7 @@ __device__ __forceinline__ void macro_ueg_wave(const Tables &T, double p, int m,
 #pragma unroll
 	for (int c = 0; c < 5; c++)
 		macro[c] = 0.0;
-	const double *__restrict__ cl = T.concs + m * T.max_num_nucs;
+	const const_table_ptr cl = as_table(T.concs) + m * T.max_num_nucs;
 	ueg_wave_rows<CHUNK>(T, m, idx, [&](int j, int nuc, int k) {
 		ueg_wave_nuclide<SHARE3, true>(T, nuc, k, p, cl[j], macro);
 	});

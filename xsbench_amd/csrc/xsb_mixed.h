@@ -168,15 +168,19 @@ __device__ __forceinline__ void mixed_term(double e_lo, const float xl[5], doubl
 // E_lo)), xs_hi[5], xs_lo[5]}: the FP64 fraction is the numerator times the
 // rounded reciprocal, within 3 u64 (relative) of the quotient; the XS are the
 // points' own FP32 values.
+// (The records are read through the constant address space, as_table: a
+// record shared by the wave comes through the scalar cache.)
 __device__ __forceinline__ double ivl32_fraction(const double *__restrict__ R, double p)
 {
-	return (R[0] - p) * R[1];
+	const const_table_ptr r = as_table(R);
+	return (r[0] - p) * r[1];
 }
 
 __device__ __forceinline__ void ivl32_terms(const double *__restrict__ R, float fc, float gc,
                                             float t[5])
 {
-	const float *f = reinterpret_cast<const float *>(R + 2);  // xh[5], xl[5]
+	typedef const __attribute__((address_space(4))) float *const_float_ptr;
+	const const_float_ptr f = (const_float_ptr)as_table(R + 2);  // xh[5], xl[5]
 #pragma unroll
 	for (int k = 0; k < 5; k++)
 		t[k] = __builtin_fmaf(f[5 + k], fc, f[k] * gc);
@@ -276,11 +280,13 @@ __device__ __forceinline__ int mixed_finish(const double s[5], float A, float wm
 	return am;
 }
 
+typedef const __attribute__((address_space(4))) float *const_float_table_ptr;
+
 template <int PREC>
-__device__ __forceinline__ const float *mixed_consts(const Tables &T, int mat)
+__device__ __forceinline__ const_float_table_ptr mixed_consts(const Tables &T, int mat)
 {
 	const int plane = T.max_num_nucs * kMaterials;
-	return T.concs32 + (PREC == XSB_XS_FP32 ? 0 : 2) * plane + mat * T.max_num_nucs;
+	return (const_float_table_ptr)T.concs32 + (PREC == XSB_XS_FP32 ? 0 : 2) * plane + mat * T.max_num_nucs;
 }
 
 // calculate_macro_xs on the mixed tables, one lane per lookup.
@@ -291,8 +297,8 @@ __device__ __forceinline__ int macro_xs_mixed(const Tables &T, double p, int mat
 {
 	constexpr int REC = (PREC == XSB_XS_FP32) ? kRec32Doubles : kRec16Doubles;
 	const int nn = T.num_nucs[mat];
-	const float *__restrict__ ceff = mixed_consts<PREC>(T, mat);
-	const float *__restrict__ cbnd = ceff + T.max_num_nucs * kMaterials;
+	const const_float_table_ptr ceff = mixed_consts<PREC>(T, mat);
+	const const_float_table_ptr cbnd = ceff + T.max_num_nucs * kMaterials;
 	const int32_t *__restrict__ ml = T.mats + mat * T.max_num_nucs;
 	const uint8_t *__restrict__ mrow = (GRID == kUnionized) ? ueg_row(T, mat, idx) : nullptr;
 	double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -356,6 +362,7 @@ __device__ __forceinline__ void ueg_wave_nuclide_mixed(const Tables &T, int nuc,
 		const double *R = T.ivl32 + ((int64_t)nuc * n_gp + k0) * kIvl32Doubles;
 		if (__ballot(du != 0u) == 0ULL) {
 			mixed_term_ivl(R, p, c, cb, sum, A, wmin);
+			__builtin_amdgcn_sched_barrier(0);  // not merged with the per-lane call
 		} else if (SHARE3 && (__ballot(du > 1u) == 0ULL || __ballot(du + 1u > 1u) == 0ULL)) {
 			// two records shared; each lane takes its own
 			const bool back = __ballot(du > 1u) != 0ULL;
@@ -408,8 +415,8 @@ __device__ __forceinline__ int macro_ueg_wave_mixed(const Tables &T, double p, i
                                                     double macro[5], uint32_t *fell_back,
                                                     double *bound)
 {
-	const float *__restrict__ ceff = mixed_consts<PREC>(T, m);
-	const float *__restrict__ cbnd = ceff + T.max_num_nucs * kMaterials;
+	const const_float_table_ptr ceff = mixed_consts<PREC>(T, m);
+	const const_float_table_ptr cbnd = ceff + T.max_num_nucs * kMaterials;
 	double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 	float A = 0.f, wmin = 1.f;
 	ueg_wave_rows<CHUNK>(T, m, idx, [&](int j, int nuc, int k) {
@@ -429,8 +436,8 @@ __device__ __forceinline__ int macro_hash_wave_mixed(const Tables &T, double p, 
                                                      double *bound)
 {
 	constexpr int REC = (PREC == XSB_XS_FP32) ? kRec32Doubles : kRec16Doubles;
-	const float *__restrict__ ceff = mixed_consts<PREC>(T, m);
-	const float *__restrict__ cbnd = ceff + T.max_num_nucs * kMaterials;
+	const const_float_table_ptr ceff = mixed_consts<PREC>(T, m);
+	const const_float_table_ptr cbnd = ceff + T.max_num_nucs * kMaterials;
 	double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 	float A = 0.f, wmin = 1.f;
 	hash_wave_rows<REC>(T, mixed_row<PREC>(T, 0), p, m, idx, [&](int j, int nuc, int k) {
