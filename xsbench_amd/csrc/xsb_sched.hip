@@ -397,6 +397,11 @@ __global__ void k_digit_starts(const uint32_t *__restrict__ hist, int ntiles, ui
 // wave-wide search and the wave-uniform material path.
 // Eight waves per SIMD (at most 64 VGPRs, a few spilled) hide the per-nuclide
 // latency chain better than the six the register demand would otherwise give.
+// On the unionized grid the wave path runs without its two-interval step:
+// with the tables read through the scalar cache, the straddling waves'
+// per-lane record loads cost less than interpolating two shared records on
+// every lane.  The hash grid's wave path keeps it (its per-lane step also
+// searches the bin per lane).
 template <int GRID, int PREC, bool DETAIL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_lookup_ids(Tables T, int64_t n,
                                                     const uint32_t *__restrict__ ids,
@@ -414,8 +419,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 	double bnd = 0.0;
 	uint32_t fb = 0;
 	int64_t idx;
-	const int am = sorted_lookup<GRID, PREC, kIdxChunk>(T, p, mat, T.variant & kVariantWave, idx,
-	                                                    macro, fb, DETAIL ? &bnd : nullptr);
+	const int am = sorted_lookup<GRID, PREC, kIdxChunk, false, GRID != kUnionized>(
+	    T, p, mat, T.variant & kVariantWave, idx, macro, fb, DETAIL ? &bnd : nullptr);
 	if (!valid)
 		fb = 0;
 	const uint32_t v = valid ? (uint32_t)am + 1u : 0u;
