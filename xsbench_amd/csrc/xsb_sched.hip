@@ -325,6 +325,7 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ ki
 constexpr int kRunsBlock = 1024;                 // 16 waves: two blocks fill a CU
 constexpr int kRunsItems = kTile / kRunsBlock;   // 8 elements per thread
 static_assert(kBins == kRunsBlock, "one digit per thread in the local scan");
+constexpr unsigned kScatterXcdRun = 16;          // tiles per XCD run
 
 __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__restrict__ pin,
                                                              uint2 *__restrict__ pout, int64_t n,
@@ -337,7 +338,10 @@ __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__rest
 	__shared__ uint32_t sk[kTile], sv[kTile];
 	cnt[threadIdx.x] = 0;
 	__syncthreads();
-	const int64_t base = (int64_t)blockIdx.x * kTile;
+	// runs of neighbouring tiles on one XCD: their runs of a digit are
+	// neighbours in the output, so partial lines merge in that XCD's L2
+	const unsigned tile = xcd_block(kScatterXcdRun);
+	const int64_t base = (int64_t)tile * kTile;
 	const int m = (int)min((int64_t)kTile, n - base);
 	uint32_t kk[kRunsItems], vv[kRunsItems], rank[kRunsItems];
 #pragma unroll
@@ -360,7 +364,7 @@ __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__rest
 	const uint32_t c = cnt[d];
 	const uint32_t run = block_excl_scan_n<kRunsBlock>(c);
 	cnt[d] = run;
-	gbase[d] = hist[(int64_t)d * ntiles + blockIdx.x] - run;
+	gbase[d] = hist[(int64_t)d * ntiles + tile] - run;
 	__syncthreads();
 #pragma unroll
 	for (int it = 0; it < kRunsItems; it++) {
