@@ -306,7 +306,8 @@ __global__ __launch_bounds__(kSortBlock) void k_bkeys(Tables T, BatchView B, int
 		h[d] = 0;
 	__syncthreads();
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const int64_t base = (int64_t)blockIdx.x * kTile;
+	const int tile = sort_tile();
+	const int64_t base = (int64_t)tile * kTile;
 	const uint32_t emax = (1u << ebits) - 1u;
 	for (int it = 0; it < kSortItems; it++) {
 		const int64_t e = base + tile_pos(wave, it, lane);
@@ -328,7 +329,7 @@ __global__ __launch_bounds__(kSortBlock) void k_bkeys(Tables T, BatchView B, int
 		}
 	}
 	__syncthreads();
-	write_tile_hist(h, hist, ntiles);
+	write_tile_hist(h, hist, ntiles, tile);
 }
 
 // Adds the block's LDS sums of batches [0, nb) to the device totals and

@@ -47,7 +47,8 @@ __global__ __launch_bounds__(kSortBlock) void k_hkeys(Tables T, int64_t P,
 		h[d] = 0;
 	__syncthreads();
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const int64_t base = (int64_t)blockIdx.x * kTile;
+	const int tile = sort_tile();
+	const int64_t base = (int64_t)tile * kTile;
 	for (int it = 0; it < kSortItems; it++) {
 		const int64_t e = base + tile_pos(wave, it, lane);
 		if (e < P) {
@@ -62,7 +63,7 @@ __global__ __launch_bounds__(kSortBlock) void k_hkeys(Tables T, int64_t P,
 		}
 	}
 	__syncthreads();
-	write_tile_hist(h, hist, ntiles);
+	write_tile_hist(h, hist, ntiles, tile);
 }
 
 // One round in sorted order (Simulation.c:176-234 for one i): draw the

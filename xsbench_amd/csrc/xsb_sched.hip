@@ -47,7 +47,8 @@ __global__ __launch_bounds__(kSortBlock) void k_keys(Tables T, int64_t first, in
 		h[d] = 0;
 	__syncthreads();
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const int64_t base = (int64_t)blockIdx.x * kTile;
+	const int tile = sort_tile();
+	const int64_t base = (int64_t)tile * kTile;
 	// A wave's lanes hold consecutive lookups (tile_pos), two draws each:
 	// the wave's first state s0 is one full skip-ahead, a lane's state is s0
 	// advanced 2 * lane draws and the next item's s0 is 128 draws on, both
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(kSortBlock) void k_keys(Tables T, int64_t first, in
 		s0 = (item_skip.x * s0 + item_skip.y) & kLcgMask;
 	}
 	__syncthreads();
-	write_tile_hist(h, hist, ntiles);
+	write_tile_hist(h, hist, ntiles, tile);
 }
 
 template <typename K>
@@ -80,7 +81,8 @@ __global__ __launch_bounds__(kSortBlock) void k_hist(const K *__restrict__ keys,
 		h[d] = 0;
 	__syncthreads();
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const int64_t base = (int64_t)blockIdx.x * kTile;
+	const int tile = sort_tile();
+	const int64_t base = (int64_t)tile * kTile;
 	uint32_t dg[kSortItems];  // all loads in flight before the counting
 #pragma unroll
 	for (int it = 0; it < kSortItems; it++) {
@@ -92,7 +94,7 @@ __global__ __launch_bounds__(kSortBlock) void k_hist(const K *__restrict__ keys,
 		if (dg[it] < (uint32_t)kBins)
 			atomicAdd(&h[dg[it]], 1u);
 	__syncthreads();
-	write_tile_hist(h, hist, ntiles);
+	write_tile_hist(h, hist, ntiles, tile);
 }
 
 // k_hist over interleaved {key, id} pairs.
@@ -105,7 +107,8 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_pairs(const uint2 *__restri
 		h[d] = 0;
 	__syncthreads();
 	// a histogram does not depend on the order: two pairs per 16-byte load
-	const int64_t base = (int64_t)blockIdx.x * kTile;
+	const int tile = sort_tile();
+	const int64_t base = (int64_t)tile * kTile;
 	const uint4 *__restrict__ q = reinterpret_cast<const uint4 *>(pairs + base);
 	uint32_t dg[kSortItems];
 #pragma unroll
@@ -124,7 +127,7 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_pairs(const uint2 *__restri
 		if (dg[it] < (uint32_t)kBins)
 			atomicAdd(&h[dg[it]], 1u);
 	__syncthreads();
-	write_tile_hist(h, hist, ntiles);
+	write_tile_hist(h, hist, ntiles, tile);
 }
 
 // ---- exclusive scan of the digit-major histogram (reduce, scan, apply) ----
@@ -272,7 +275,8 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ ki
 	__syncthreads();
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
-	const int64_t base = (int64_t)blockIdx.x * kTile;
+	const int tile = sort_tile();
+	const int64_t base = (int64_t)tile * kTile;
 	K kk[kSortItems];
 	uint32_t vv[kSortItems];
 	uint16_t loc[kSortItems];
@@ -297,7 +301,7 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ ki
 	}
 	__syncthreads();
 	for (int d = threadIdx.x; d < kBins; d += kSortBlock) {
-		uint32_t run = hist[(int64_t)d * ntiles + blockIdx.x];
+		uint32_t run = hist[(int64_t)d * ntiles + tile];
 		for (int w = 0; w < kSortWaves; w++) {
 			const uint32_t t = cnt[w][d];
 			cnt[w][d] = run;
@@ -325,7 +329,6 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ ki
 constexpr int kRunsBlock = 1024;                 // 16 waves: two blocks fill a CU
 constexpr int kRunsItems = kTile / kRunsBlock;   // 8 elements per thread
 static_assert(kBins == kRunsBlock, "one digit per thread in the local scan");
-constexpr unsigned kScatterXcdRun = 16;          // tiles per XCD run
 
 __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__restrict__ pin,
                                                              uint2 *__restrict__ pout, int64_t n,
@@ -338,9 +341,7 @@ __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__rest
 	__shared__ uint32_t sk[kTile], sv[kTile];
 	cnt[threadIdx.x] = 0;
 	__syncthreads();
-	// runs of neighbouring tiles on one XCD: their runs of a digit are
-	// neighbours in the output, so partial lines merge in that XCD's L2
-	const unsigned tile = xcd_block(kScatterXcdRun);
+	const int tile = sort_tile();
 	const int64_t base = (int64_t)tile * kTile;
 	const int m = (int)min((int64_t)kTile, n - base);
 	uint32_t kk[kRunsItems], vv[kRunsItems], rank[kRunsItems];

@@ -28,6 +28,13 @@ __device__ __forceinline__ int tile_pos(int wave, int item, int lane)
 
 inline int sort_tiles(int64_t n) { return (int)((n + kTile - 1) / kTile); }
 
+// The tile a sort kernel's workgroup takes: runs of kSortXcdRun neighbouring
+// tiles per XCD (xcd_block), so that what neighbouring tiles write next to
+// each other (histogram columns, a digit's output runs) merges into whole
+// lines in one L2 before it reaches HBM.
+constexpr unsigned kSortXcdRun = 16;
+__device__ __forceinline__ int sort_tile() { return (int)xcd_block(kSortXcdRun); }
+
 // Bytes of work memory the sorted event schedules take for n lookups: two
 // buffers of {key, id} pairs (16-byte aligned), the per-tile digit histogram,
 // the scan partials, and the last pass's digit starts.
@@ -62,10 +69,11 @@ int sort_pairs64(hipStream_t s, int64_t n, int passes, uint64_t **k, uint32_t **
 
 // Per-tile first-digit histogram from inside a key kernel: h is the block's
 // LDS histogram (kBins words), zeroed by the caller.
-__device__ __forceinline__ void write_tile_hist(const uint32_t *h, uint32_t *hist, int ntiles)
+__device__ __forceinline__ void write_tile_hist(const uint32_t *h, uint32_t *hist, int ntiles,
+                                                int tile)
 {
 	for (int d = threadIdx.x; d < kBins; d += kSortBlock)
-		hist[(int64_t)d * ntiles + blockIdx.x] = h[d];
+		hist[(int64_t)d * ntiles + tile] = h[d];
 }
 
 // ---- lookup body in sorted order ----
