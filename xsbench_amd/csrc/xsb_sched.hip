@@ -330,6 +330,9 @@ constexpr int kRunsBlock = 1024;                 // 16 waves: two blocks fill a 
 constexpr int kRunsItems = kTile / kRunsBlock;   // 8 elements per thread
 static_assert(kBins == kRunsBlock, "one digit per thread in the local scan");
 
+// IDS: the pass writes only the ids, packed (uint32_t[n] at pout), for a
+// caller that needs the order and not the keys.
+template <bool IDS>
 __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__restrict__ pin,
                                                              uint2 *__restrict__ pout, int64_t n,
                                                              int shift,
@@ -383,7 +386,10 @@ __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__rest
 		if (i < m) {
 			const uint32_t key = sk[i];
 			const uint32_t g = gbase[(key >> shift) & kDigitMask] + (uint32_t)i;
-			pout[g] = make_uint2(key, sv[i]);
+			if (IDS)
+				reinterpret_cast<uint32_t *>(pout)[g] = sv[i];
+			else
+				pout[g] = make_uint2(key, sv[i]);
 		}
 	}
 }
@@ -395,8 +401,8 @@ __global__ void k_digit_starts(const uint32_t *__restrict__ hist, int ntiles, ui
 		out[d] = hist[(int64_t)d * ntiles];
 }
 
-// Lookups in sorted order: position i holds lookup id ids[2 i] (the ids of
-// the sorted {key, id} pairs); (E, mat) are
+// Lookups in sorted order: position i holds lookup id ids[i] (the sort's
+// last pass writes the ids alone); (E, mat) are
 // re-derived from the id's LCG state.  Lanes past n repeat lookup n-1 (in the
 // same wave) and contribute nothing, so every lane stays active for the
 // wave-wide search and the wave-uniform material path.
@@ -416,7 +422,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 {
 	const int64_t i = (int64_t)xcd_block(kXcdRun) * 256 + threadIdx.x;
 	const bool valid = i < n;
-	const uint32_t id = ids[(valid ? i : n - 1) * 2];  // ids interleaved with keys
+	const uint32_t id = ids[valid ? i : n - 1];
 	uint64_t s = ff_jump(seed0, 2ULL * (uint64_t)(first + id), T.ff_tab);
 	const double p = lcg_double(s);
 	const int mat = pick_mat(s, T.thr);
@@ -518,7 +524,7 @@ int sort_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint32_t 
 }
 
 int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint2 **p,
-                            uint2 **p2, uint32_t *hist, uint32_t *part)
+                            uint2 **p2, uint32_t *hist, uint32_t *part, bool ids_last)
 {
 	const int ntiles = sort_tiles(n);
 	const int64_t hlen = sort_hist_len(n);
@@ -536,8 +542,12 @@ int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint2 **
 		}
 		if ((rc = scan_hist(hist, hlen, part, s)))
 			return rc;
-		hipLaunchKernelGGL(k_scatter_runs, dim3(ntiles), dim3(kRunsBlock), 0, s, *p, *p2, n, shift,
-		                   hist, ntiles);
+		if (ids_last && pass == passes - 1)
+			hipLaunchKernelGGL(k_scatter_runs<true>, dim3(ntiles), dim3(kRunsBlock), 0, s, *p, *p2,
+			                   n, shift, hist, ntiles);
+		else
+			hipLaunchKernelGGL(k_scatter_runs<false>, dim3(ntiles), dim3(kRunsBlock), 0, s, *p, *p2,
+			                   n, shift, hist, ntiles);
 		XSB_HIP(hipGetLastError());
 		uint2 *t = *p;
 		*p = *p2;
@@ -588,9 +598,9 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	hipLaunchKernelGGL(k_keys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, a->first_lookup, n,
 	                   a->seed, mode, pA, hist, ntiles);
 	XSB_HIP(hipGetLastError());
-	if ((rc = order_pairs(s, n, passes, true, &pA, &pB, hist, part)))
+	if ((rc = order_pairs(s, n, passes, true, &pA, &pB, hist, part, true)))
 		return rc;
-	const uint32_t *vA = reinterpret_cast<const uint32_t *>(pA) + 1;
+	const uint32_t *vA = reinterpret_cast<const uint32_t *>(pA);  // the ids, packed
 	hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(256), 0, s, hist, ntiles, starts);
 	XSB_HIP(hipGetLastError());
 	XSB_HIP(hipEventRecord(d->ev[1], s));
