@@ -198,36 +198,20 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t *__restrict_
 		part[blockIdx.x] = tot;
 }
 
-// single block: exclusive scan of the partials, kScanChunk at a time
-__global__ __launch_bounds__(256) void k_scan_parts(uint32_t *part, int nparts)
-{
-	uint32_t carry = 0;
-	for (int base = 0; base < nparts; base += kScanChunk) {
-		uint32_t v[kScanItems];
-		uint32_t s = 0;
-		for (int it = 0; it < kScanItems; it++) {
-			const int i = base + threadIdx.x * kScanItems + it;
-			v[it] = i < nparts ? part[i] : 0u;
-			s += v[it];
-		}
-		uint32_t tot;
-		uint32_t off = block_excl_scan(s, &tot) + carry;
-		for (int it = 0; it < kScanItems; it++) {
-			const int i = base + threadIdx.x * kScanItems + it;
-			if (i < nparts)
-				part[i] = off;
-			off += v[it];
-		}
-		carry += tot;
-	}
-}
-
-// The histogram length is a multiple of kBins, so a thread's 16 words are
-// either all in range or all out: read and written as four 16-byte vectors.
+// Each block first adds up the chunk totals of the blocks before it (a few
+// loads per thread: the totals are few and cached), so no separate pass
+// scans them.  The histogram length is a multiple of kBins, so a thread's
+// 16 words are either all in range or all out: read and written as four
+// 16-byte vectors.
 static_assert(kScanItems == 16 && kBins % kScanItems == 0, "whole vectors per thread");
 __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *a, int64_t n,
                                                     const uint32_t *__restrict__ part)
 {
+	uint32_t c = 0;
+	for (int i = threadIdx.x; i < (int)blockIdx.x; i += 256)
+		c += part[i];
+	uint32_t carry;
+	block_excl_scan(c, &carry);
 	const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
 	const bool in = base < n;
 	uint4 w[kScanItems / 4];
@@ -238,7 +222,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(uint32_t *a, int64_t n,
 #pragma unroll
 	for (int q = 0; q < kScanItems / 4; q++)
 		s += w[q].x + w[q].y + w[q].z + w[q].w;
-	uint32_t off = block_excl_scan(s, nullptr) + part[blockIdx.x];
+	uint32_t off = block_excl_scan(s, nullptr) + carry;
 	if (!in)
 		return;
 #pragma unroll
@@ -331,13 +315,14 @@ constexpr int kRunsItems = kTile / kRunsBlock;   // 8 elements per thread
 static_assert(kBins == kRunsBlock, "one digit per thread in the local scan");
 
 // IDS: the pass writes only the ids, packed (uint32_t[n] at pout), for a
-// caller that needs the order and not the keys.
+// caller that needs the order and not the keys.  starts (if given) gets each
+// digit's first position in the output.
 template <bool IDS>
 __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__restrict__ pin,
                                                              uint2 *__restrict__ pout, int64_t n,
                                                              int shift,
                                                              const uint32_t *__restrict__ hist,
-                                                             int ntiles)
+                                                             int ntiles, uint32_t *starts)
 {
 	__shared__ uint32_t cnt[kBins];   // digit counts, then the tile's local starts
 	__shared__ uint32_t gbase[kBins]; // global start minus local start, per digit
@@ -368,7 +353,10 @@ __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__rest
 	const uint32_t c = cnt[d];
 	const uint32_t run = block_excl_scan_n<kRunsBlock>(c);
 	cnt[d] = run;
-	gbase[d] = hist[(int64_t)d * ntiles + tile] - run;
+	const uint32_t start = hist[(int64_t)d * ntiles + tile];
+	if (starts && tile == 0)
+		starts[d] = start;
+	gbase[d] = start - run;
 	__syncthreads();
 #pragma unroll
 	for (int it = 0; it < kRunsItems; it++) {
@@ -394,12 +382,6 @@ __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__rest
 	}
 }
 
-// first element of each digit after the final pass (for material counts)
-__global__ void k_digit_starts(const uint32_t *__restrict__ hist, int ntiles, uint32_t *out)
-{
-	for (int d = threadIdx.x; d < kBins; d += blockDim.x)
-		out[d] = hist[(int64_t)d * ntiles];
-}
 
 // Lookups in sorted order: position i holds lookup id ids[i] (the sort's
 // last pass writes the ids alone); (E, mat) are
@@ -480,7 +462,6 @@ static int scan_hist(uint32_t *hist, int64_t len, uint32_t *part, hipStream_t s)
 {
 	const int nparts = (int)((len + kScanChunk - 1) / kScanChunk);
 	hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(256), 0, s, hist, len, part);
-	hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(256), 0, s, part, nparts);
 	hipLaunchKernelGGL(k_scan_apply, dim3(nparts), dim3(256), 0, s, hist, len, part);
 	XSB_HIP(hipGetLastError());
 	return XSB_OK;
@@ -524,7 +505,8 @@ int sort_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint32_t 
 }
 
 int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint2 **p,
-                            uint2 **p2, uint32_t *hist, uint32_t *part, bool ids_last)
+                            uint2 **p2, uint32_t *hist, uint32_t *part, bool ids_last,
+                            uint32_t *starts)
 {
 	const int ntiles = sort_tiles(n);
 	const int64_t hlen = sort_hist_len(n);
@@ -542,12 +524,13 @@ int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint2 **
 		}
 		if ((rc = scan_hist(hist, hlen, part, s)))
 			return rc;
-		if (ids_last && pass == passes - 1)
+		const bool last = pass == passes - 1;
+		if (ids_last && last)
 			hipLaunchKernelGGL(k_scatter_runs<true>, dim3(ntiles), dim3(kRunsBlock), 0, s, *p, *p2,
-			                   n, shift, hist, ntiles);
+			                   n, shift, hist, ntiles, starts);
 		else
 			hipLaunchKernelGGL(k_scatter_runs<false>, dim3(ntiles), dim3(kRunsBlock), 0, s, *p, *p2,
-			                   n, shift, hist, ntiles);
+			                   n, shift, hist, ntiles, last ? starts : nullptr);
 		XSB_HIP(hipGetLastError());
 		uint2 *t = *p;
 		*p = *p2;
@@ -598,11 +581,9 @@ int run_sorted_event(Device *d, const xsb_event_args *a, const DetailOut *det, x
 	hipLaunchKernelGGL(k_keys, dim3(ntiles), dim3(kSortBlock), 0, s, d->T, a->first_lookup, n,
 	                   a->seed, mode, pA, hist, ntiles);
 	XSB_HIP(hipGetLastError());
-	if ((rc = order_pairs(s, n, passes, true, &pA, &pB, hist, part, true)))
+	if ((rc = order_pairs(s, n, passes, true, &pA, &pB, hist, part, true, starts)))
 		return rc;
 	const uint32_t *vA = reinterpret_cast<const uint32_t *>(pA);  // the ids, packed
-	hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(256), 0, s, hist, ntiles, starts);
-	XSB_HIP(hipGetLastError());
 	XSB_HIP(hipEventRecord(d->ev[1], s));
 	if (a->xs_precision == XSB_XS_FP64)
 		launch_ids_prec<XSB_XS_FP64>(d, s, n, vA, a->first_lookup, a->seed, det, vsum, nfb);

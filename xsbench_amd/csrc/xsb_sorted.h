@@ -61,9 +61,11 @@ int sort_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint32_t 
 // element): sorted by key, but elements with equal keys may be permuted
 // within a tile of kTile (faster: coalesced runs instead of a stable
 // per-wave ranking).  *p points at the ordered pairs on return, or with
-// ids_last at the ordered ids alone (uint32_t[n]).
+// ids_last at the ordered ids alone (uint32_t[n]).  starts, if given, gets
+// the last pass's digit starts (kBins words).
 int order_pairs(hipStream_t s, int64_t n, int passes, bool hist0_ready, uint2 **p, uint2 **p2,
-                uint32_t *hist, uint32_t *part, bool ids_last = false);
+                uint32_t *hist, uint32_t *part, bool ids_last = false,
+                uint32_t *starts = nullptr);
 // The same for 64-bit keys (no key-kernel histogram).
 int sort_pairs64(hipStream_t s, int64_t n, int passes, uint64_t **k, uint32_t **v, uint64_t **k2,
                  uint32_t **v2, uint32_t *hist, uint32_t *part);
