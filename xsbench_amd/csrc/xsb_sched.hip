@@ -16,7 +16,8 @@
 namespace xsb {
 
 constexpr int kEnergyBits = 16;
-constexpr unsigned kXcdRun = 16;  // lookup blocks per XCD run
+constexpr int kIdsBlock = 1024;   // lookup kernel workgroup
+constexpr unsigned kXcdRun = 16 * 256 / kIdsBlock;  // lookup blocks per XCD run (4096 lookups)
 
 enum SortMode : int { kModeMat = 0, kModeFuel = 1, kModeMatE = 2 };
 
@@ -396,13 +397,13 @@ __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__rest
 // every lane.  The hash grid's wave path keeps it (its per-lane step also
 // searches the bin per lane).
 template <int GRID, int PREC, bool DETAIL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_lookup_ids(Tables T, int64_t n,
+__global__ __launch_bounds__(kIdsBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_lookup_ids(Tables T, int64_t n,
                                                     const uint32_t *__restrict__ ids,
                                                     int64_t first, uint64_t seed0, DetailOut det,
                                                     unsigned long long *vsum,
                                                     unsigned long long *nfb)
 {
-	const int64_t i = (int64_t)xcd_block(kXcdRun) * 256 + threadIdx.x;
+	const int64_t i = (int64_t)xcd_block(kXcdRun) * kIdsBlock + threadIdx.x;
 	const bool valid = i < n;
 	const uint32_t id = ids[valid ? i : n - 1];
 	uint64_t s = ff_jump(seed0, 2ULL * (uint64_t)(first + id), T.ff_tab);
@@ -428,9 +429,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 		if (det.flag) det.flag[o] = (int32_t)fb;
 		if (det.bound) det.bound[o] = bnd;
 	}
-	block_add_u64<256>(v, vsum);
+	block_add_u64<kIdsBlock>(v, vsum);
 	if (PREC != XSB_XS_FP64 && nfb)
-		block_add_u64<256>(fb, nfb);
+		block_add_u64<kIdsBlock>(fb, nfb);
 }
 
 template <int PREC>
@@ -438,16 +439,16 @@ static void launch_ids_prec(Device *d, hipStream_t s, int64_t n, const uint32_t 
                             uint64_t seed, const DetailOut *det, unsigned long long *vsum,
                             unsigned long long *nfb)
 {
-	const unsigned nb = (unsigned)((n + 255) / 256);
+	const unsigned nb = (unsigned)((n + kIdsBlock - 1) / kIdsBlock);
 	DetailOut dz{};
 	const DetailOut &dd = det ? *det : dz;
 #define XSB_LAUNCH(G)                                                                          \
 	do {                                                                                       \
 		if (det)                                                                               \
-			hipLaunchKernelGGL((k_lookup_ids<G, PREC, true>), dim3(nb), dim3(256), 0, s, d->T, \
+			hipLaunchKernelGGL((k_lookup_ids<G, PREC, true>), dim3(nb), dim3(kIdsBlock), 0, s, d->T, \
 			                   n, ids, first, seed, dd, vsum, nfb);                            \
 		else                                                                                   \
-			hipLaunchKernelGGL((k_lookup_ids<G, PREC, false>), dim3(nb), dim3(256), 0, s,      \
+			hipLaunchKernelGGL((k_lookup_ids<G, PREC, false>), dim3(nb), dim3(kIdsBlock), 0, s,      \
 			                   d->T, n, ids, first, seed, dd, vsum, nfb);                      \
 	} while (0)
 	switch (d->T.grid_type) {
