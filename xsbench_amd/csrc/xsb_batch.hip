@@ -37,6 +37,9 @@
 namespace xsb {
 
 constexpr int kBatchBlock = 256;
+// The sorted pipeline's persistent kernel (512 threads: FP64 9.65 -> 10.7 ms
+// for 122M lookups, 1024: 12.2 ms; the event kernel gains from 512).
+constexpr int kBSortedBlock = 256;
 
 __device__ __forceinline__ void batch_sample(const Tables &T, uint64_t seed, const double *thr,
                                              int64_t i, double &p, int &mat)
@@ -337,7 +340,7 @@ __global__ __launch_bounds__(kSortBlock) void k_bkeys(Tables T, BatchView B, int
 __device__ __forceinline__ void flush_batches(const BatchView &B, const SortedQueue &q,
                                               uint32_t *s_v, uint32_t *s_c)
 {
-	for (int b = threadIdx.x; b < B.nb; b += kBatchBlock) {
+	for (int b = threadIdx.x; b < B.nb; b += kBSortedBlock) {
 		const uint32_t c = s_c[b];
 		if (c) {
 			atomicAdd(q.vsum + b, (unsigned long long)s_v[b]);
@@ -354,7 +357,7 @@ __device__ __forceinline__ void flush_batches(const BatchView &B, const SortedQu
 // allocator lands on either side of 128 from one variant to the next, and
 // three waves lose about a quarter of the throughput.
 template <int GRID, int PREC, int CHUNK, bool SHARE3>
-__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_bsorted(
+__global__ __launch_bounds__(kBSortedBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_bsorted(
     Tables T, BatchView B, SortedQueue q)
 {
 	__shared__ uint32_t s_off[kLdsBatches + 1];
@@ -363,9 +366,9 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(4))
 	const int nb = B.nb;
 	const bool lds = nb <= kLdsBatches;
 	if (lds) {
-		for (int b = threadIdx.x; b <= nb; b += kBatchBlock)
+		for (int b = threadIdx.x; b <= nb; b += kBSortedBlock)
 			s_off[b] = B.off[b];
-		for (int b = threadIdx.x; b < nb; b += kBatchBlock)
+		for (int b = threadIdx.x; b < nb; b += kBSortedBlock)
 			s_v[b] = s_c[b] = 0;
 	}
 	__syncthreads();
@@ -411,7 +414,7 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(4))
 		const int64_t p0 = lo + item * q.tile, p1 = min(p0 + (int64_t)q.tile, hi);
 		// all threads run the same trip count; lanes past p1 repeat p1 - 1 and
 		// contribute nothing, so waves stay whole for the wave-wide paths
-		for (int64_t r = p0; r < p1; r += kBatchBlock) {
+		for (int64_t r = p0; r < p1; r += kBSortedBlock) {
 			const int64_t pos = r + threadIdx.x;
 			const bool valid = pos < p1;
 			const int64_t pp = valid ? pos : p1 - 1;
@@ -451,14 +454,14 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(4))
 		flush_batches(B, q, s_v, s_c);
 	}
 	if (PREC != XSB_XS_FP64)
-		block_add_u64<kBatchBlock>(fb_total, q.nfb);
+		block_add_u64<kBSortedBlock>(fb_total, q.nfb);
 }
 
 template <int GRID, int PREC, int CHUNK, bool SHARE3>
 static int bsorted_occupancy(int *blocks)
 {
 	XSB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-	    blocks, k_bsorted<GRID, PREC, CHUNK, SHARE3>, kBatchBlock, 0));
+	    blocks, k_bsorted<GRID, PREC, CHUNK, SHARE3>, kBSortedBlock, 0));
 	return XSB_OK;
 }
 
@@ -469,7 +472,7 @@ static int launch_or_size(Device *d, const BatchView &B, const SortedQueue &q, b
 	if (occupancy_only)
 		return bsorted_occupancy<GRID, PREC, CHUNK, SHARE3>(occ);
 	hipLaunchKernelGGL((k_bsorted<GRID, PREC, CHUNK, SHARE3>), dim3((unsigned)*occ),
-	                   dim3(kBatchBlock), 0, s, d->T, B, q);
+	                   dim3(kBSortedBlock), 0, s, d->T, B, q);
 	XSB_HIP(hipGetLastError());
 	return XSB_OK;
 }

@@ -20,7 +20,10 @@ namespace xsb {
 
 constexpr int kHBlock = 256;
 constexpr int kHEnergyBits = 16;
-constexpr unsigned kHXcdRun = 16;
+// Round kernel workgroup (512 threads: 30.0 -> 31.3 ms for the XL-45000
+// run, 1024: 31.6 ms).
+constexpr int kHRoundBlock = 256;
+constexpr unsigned kHXcdRun = 4096 / kHRoundBlock;
 
 __global__ __launch_bounds__(kHBlock) void k_hinit(Tables T, int64_t P, int lookups, uint64_t seed0,
                                                    uint64_t *seed, uint32_t *psum)
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(kSortBlock) void k_hkeys(Tables T, int64_t P,
 // (queue[0] counts, queue[1 ..] particles) for k_hfallback, its particle's
 // state left as it was.
 template <int GRID, int PREC, int CHUNK>
-__global__ __launch_bounds__(kHBlock) void k_hround(Tables T, int64_t P,
+__global__ __launch_bounds__(kHRoundBlock) void k_hround(Tables T, int64_t P,
                                                     const uint2 *__restrict__ ids,
                                                     uint64_t *__restrict__ seed,
                                                     uint32_t *__restrict__ psum,
@@ -85,7 +88,7 @@ __global__ __launch_bounds__(kHBlock) void k_hround(Tables T, int64_t P,
                                                     unsigned long long *nfb,
                                                     uint32_t *__restrict__ queue)
 {
-	const int64_t i = (int64_t)xcd_block(kHXcdRun) * kHBlock + threadIdx.x;
+	const int64_t i = (int64_t)xcd_block(kHXcdRun) * kHRoundBlock + threadIdx.x;
 	const bool valid = i < P;
 	const uint32_t q = ids[valid ? i : P - 1].y;
 	uint64_t s = seed[q];
@@ -109,9 +112,9 @@ __global__ __launch_bounds__(kHBlock) void k_hround(Tables T, int64_t P,
 		seed[q] = s;
 		psum[q] += (uint32_t)am + 1u;
 	}
-	block_add_u64<kHBlock>(done ? (uint32_t)am + 1u : 0u, vsum);
+	block_add_u64<kHRoundBlock>(done ? (uint32_t)am + 1u : 0u, vsum);
 	if (PREC != XSB_XS_FP64)
-		block_add_u64<kHBlock>(valid ? fb : 0u, nfb);
+		block_add_u64<kHRoundBlock>(valid ? fb : 0u, nfb);
 }
 
 // The queued lookups of a round in FP64, one workgroup each: thread t
@@ -226,9 +229,9 @@ template <int GRID, int PREC>
 static void launch_round(Device *d, hipStream_t s, int64_t P, const uint2 *ids, uint64_t *seed,
                          uint32_t *psum, unsigned long long *acc, uint32_t *queue)
 {
-	const unsigned nb = (unsigned)((P + kHBlock - 1) / kHBlock);
+	const unsigned nb = (unsigned)((P + kHRoundBlock - 1) / kHRoundBlock);
 	constexpr int CHUNK = kIdxChunk;
-	hipLaunchKernelGGL((k_hround<GRID, PREC, CHUNK>), dim3(nb), dim3(kHBlock), 0, s, d->T, P, ids,
+	hipLaunchKernelGGL((k_hround<GRID, PREC, CHUNK>), dim3(nb), dim3(kHRoundBlock), 0, s, d->T, P, ids,
 	                   seed, psum, acc, acc + 1, queue);
 	if (PREC != XSB_XS_FP64)
 		hipLaunchKernelGGL((k_hfallback<GRID>), dim3(fallback_blocks(PREC, P)), dim3(kFbBlock), 0, s,
