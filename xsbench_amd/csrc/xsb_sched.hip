@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kSortBlock) void k_hist_pairs(const uint2 *__restri
 	write_tile_hist(h, hist, ntiles, tile);
 }
 
-// ---- exclusive scan of the digit-major histogram (reduce, scan, apply) ----
+// ---- exclusive scan of the digit-major histogram (chunk totals, then apply) ----
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane)
 {
 #pragma unroll
@@ -389,14 +389,13 @@ __global__ __launch_bounds__(kRunsBlock) void k_scatter_runs(const uint2 *__rest
 	}
 }
 
-
 // Lookups in sorted order: position i holds lookup id ids[i] (the sort's
 // last pass writes the ids alone); (E, mat) are
 // re-derived from the id's LCG state.  Lanes past n repeat lookup n-1 (in the
 // same wave) and contribute nothing, so every lane stays active for the
 // wave-wide search and the wave-uniform material path.
-// Eight waves per SIMD (at most 64 VGPRs, a few spilled) hide the per-nuclide
-// latency chain better than the six the register demand would otherwise give.
+// Eight waves per SIMD (at most 64 VGPRs) hide the per-nuclide latency chain
+// better than the six the register demand would otherwise give.
 // On the unionized grid the wave path runs without its two-interval step:
 // with the tables read through the scalar cache, the straddling waves'
 // per-lane record loads cost less than interpolating two shared records on
@@ -449,14 +448,14 @@ static void launch_ids_prec(Device *d, hipStream_t s, int64_t n, const uint32_t 
 	const unsigned nb = (unsigned)((n + kIdsBlock - 1) / kIdsBlock);
 	DetailOut dz{};
 	const DetailOut &dd = det ? *det : dz;
-#define XSB_LAUNCH(G)                                                                          \
-	do {                                                                                       \
-		if (det)                                                                               \
-			hipLaunchKernelGGL((k_lookup_ids<G, PREC, true>), dim3(nb), dim3(kIdsBlock), 0, s, d->T, \
-			                   n, ids, first, seed, dd, vsum, nfb);                            \
-		else                                                                                   \
-			hipLaunchKernelGGL((k_lookup_ids<G, PREC, false>), dim3(nb), dim3(kIdsBlock), 0, s,      \
-			                   d->T, n, ids, first, seed, dd, vsum, nfb);                      \
+#define XSB_LAUNCH(G)                                                                         \
+	do {                                                                                      \
+		if (det)                                                                              \
+			hipLaunchKernelGGL((k_lookup_ids<G, PREC, true>), dim3(nb), dim3(kIdsBlock), 0, s, \
+			                   d->T, n, ids, first, seed, dd, vsum, nfb);                     \
+		else                                                                                  \
+			hipLaunchKernelGGL((k_lookup_ids<G, PREC, false>), dim3(nb), dim3(kIdsBlock), 0, \
+			                   s, d->T, n, ids, first, seed, dd, vsum, nfb);                  \
 	} while (0)
 	switch (d->T.grid_type) {
 	case kUnionized: XSB_LAUNCH(kUnionized); break;
