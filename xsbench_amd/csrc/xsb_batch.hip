@@ -280,6 +280,7 @@ struct SortedQueue {
 	int32_t ebits;
 	int32_t tile;
 	int32_t wave;              // wave-uniform lookup path for material-uniform waves
+	int32_t w_begin, w_end;    // the windows this launch works through
 	uint32_t *heads;           // [windows x 8] items taken from each (window, XCD) slice
 	unsigned long long *vsum;  // [nb]
 	unsigned long long *cnt;   // [nb] lookups completed
@@ -355,9 +356,11 @@ __device__ __forceinline__ void flush_batches(const BatchView &B, const SortedQu
 
 // At least four waves per SIMD (at most 128 VGPRs): left free, the register
 // allocator lands on either side of 128 from one variant to the next, and
-// three waves lose about a quarter of the throughput.
-template <int GRID, int PREC, int CHUNK, bool SHARE3>
-__global__ __launch_bounds__(kBSortedBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_bsorted(
+// three waves lose about a quarter of the throughput.  DENSE: eight waves
+// (64 VGPRs, the per-lane path spilling), for windows dense enough that the
+// wave-uniform path dominates (first_dense_window).
+template <int GRID, int PREC, int CHUNK, bool SHARE3, bool DENSE>
+__global__ __launch_bounds__(kBSortedBlock) __attribute__((amdgpu_waves_per_eu(DENSE ? 8 : 4))) void k_bsorted(
     Tables T, BatchView B, SortedQueue q)
 {
 	__shared__ uint32_t s_off[kLdsBatches + 1];
@@ -384,9 +387,9 @@ __global__ __launch_bounds__(kBSortedBlock) __attribute__((amdgpu_waves_per_eu(4
 	// spread over its whole window in sorted order, so its last ones are in
 	// the window's last items, after which their workgroups move on (a flush
 	// after every item stamped the same completions and cost 5% in atomics).
-	for (int w = 0, xq = home, tries = 0; w < q.windows;) {
+	for (int w = q.w_begin, xq = home, tries = 0; w < q.w_end;) {
 		if (tries == kXcds) {
-			if (lds && w + 1 < q.windows) {
+			if (lds && w + 1 < q.w_end) {
 				__syncthreads();
 				flush_batches(B, q, s_v, s_c);
 				since_flush = 0;
@@ -457,66 +460,78 @@ __global__ __launch_bounds__(kBSortedBlock) __attribute__((amdgpu_waves_per_eu(4
 		block_add_u64<kBSortedBlock>(fb_total, q.nfb);
 }
 
-template <int GRID, int PREC, int CHUNK, bool SHARE3>
-static int bsorted_occupancy(int *blocks)
-{
-	XSB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-	    blocks, k_bsorted<GRID, PREC, CHUNK, SHARE3>, kBSortedBlock, 0));
-	return XSB_OK;
-}
-
-template <int GRID, int PREC, int CHUNK, bool SHARE3>
+template <int GRID, int PREC, int CHUNK, bool SHARE3, bool DENSE>
 static int launch_or_size(Device *d, const BatchView &B, const SortedQueue &q, bool occupancy_only,
                           int *occ, hipStream_t s)
 {
-	if (occupancy_only)
-		return bsorted_occupancy<GRID, PREC, CHUNK, SHARE3>(occ);
-	hipLaunchKernelGGL((k_bsorted<GRID, PREC, CHUNK, SHARE3>), dim3((unsigned)*occ),
+	if (occupancy_only) {
+		XSB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+		    occ, k_bsorted<GRID, PREC, CHUNK, SHARE3, DENSE>, kBSortedBlock, 0));
+		return XSB_OK;
+	}
+	hipLaunchKernelGGL((k_bsorted<GRID, PREC, CHUNK, SHARE3, DENSE>), dim3((unsigned)*occ),
 	                   dim3(kBSortedBlock), 0, s, d->T, B, q);
 	XSB_HIP(hipGetLastError());
 	return XSB_OK;
 }
 
-// dispatch on (grid type, precision, staging depth, two-interval step); the
-// staging depth only matters on the unionized grid
+template <int GRID, int PREC, int CHUNK, bool SHARE3>
+static int launch_or_size_dense(Device *d, const BatchView &B, const SortedQueue &q,
+                                bool dense, bool occupancy_only, int *occ, hipStream_t s)
+{
+	if (GRID == kUnionized && dense)
+		return launch_or_size<GRID, PREC, CHUNK, SHARE3, true>(d, B, q, occupancy_only, occ, s);
+	return launch_or_size<GRID, PREC, CHUNK, SHARE3, false>(d, B, q, occupancy_only, occ, s);
+}
+
+// dispatch on (grid type, precision, staging depth, two-interval step, dense
+// windows); the staging depth and the dense variant only matter on the
+// unionized grid
 template <int GRID, int PREC>
 static int dispatch_bsorted(Device *d, const BatchView &B, const SortedQueue &q, int stage,
-                            bool share3, bool occupancy_only, int *occ, hipStream_t s)
+                            bool share3, bool dense, bool occupancy_only, int *occ, hipStream_t s)
 {
-	if (GRID != kUnionized)
-		return share3 ? launch_or_size<GRID, PREC, 16, true>(d, B, q, occupancy_only, occ, s)
-		              : launch_or_size<GRID, PREC, 16, false>(d, B, q, occupancy_only, occ, s);
-	if (stage >= 16)
-		return share3 ? launch_or_size<GRID, PREC, 16, true>(d, B, q, occupancy_only, occ, s)
-		              : launch_or_size<GRID, PREC, 16, false>(d, B, q, occupancy_only, occ, s);
-	return share3 ? launch_or_size<GRID, PREC, 8, true>(d, B, q, occupancy_only, occ, s)
-	              : launch_or_size<GRID, PREC, 8, false>(d, B, q, occupancy_only, occ, s);
+	if (GRID != kUnionized || stage >= 16)
+		return share3 ? launch_or_size_dense<GRID, PREC, 16, true>(d, B, q, dense,
+		                                                           occupancy_only, occ, s)
+		              : launch_or_size_dense<GRID, PREC, 16, false>(d, B, q, dense,
+		                                                            occupancy_only, occ, s);
+	return share3 ? launch_or_size_dense<GRID, PREC, 8, true>(d, B, q, dense, occupancy_only,
+	                                                          occ, s)
+	              : launch_or_size_dense<GRID, PREC, 8, false>(d, B, q, dense, occupancy_only,
+	                                                           occ, s);
 }
 
 template <int GRID>
 static int dispatch_bsorted_prec(Device *d, int prec, const BatchView &B, const SortedQueue &q,
-                                 int stage, bool share3, bool occupancy_only, int *occ,
-                                 hipStream_t s)
+                                 int stage, bool share3, bool dense, bool occupancy_only,
+                                 int *occ, hipStream_t s)
 {
 	if (prec == XSB_XS_FP64)
-		return dispatch_bsorted<GRID, XSB_XS_FP64>(d, B, q, stage, share3, occupancy_only, occ, s);
+		return dispatch_bsorted<GRID, XSB_XS_FP64>(d, B, q, stage, share3, dense, occupancy_only,
+		                                            occ, s);
 	if (prec == XSB_XS_FP32)
-		return dispatch_bsorted<GRID, XSB_XS_FP32>(d, B, q, stage, share3, occupancy_only, occ, s);
-	return dispatch_bsorted<GRID, XSB_XS_BF16>(d, B, q, stage, share3, occupancy_only, occ, s);
+		return dispatch_bsorted<GRID, XSB_XS_FP32>(d, B, q, stage, share3, dense, occupancy_only,
+		                                            occ, s);
+	return dispatch_bsorted<GRID, XSB_XS_BF16>(d, B, q, stage, share3, dense, occupancy_only, occ,
+	                                            s);
 }
 
 // occupancy_only: *occ receives resident workgroups per CU; else *occ is
-// the grid size to launch
+// the grid size to launch.  dense: the eight-wave variant (unionized grid).
 static int run_bsorted(Device *d, int prec, const BatchView &B, const SortedQueue &q, int stage,
-                       bool share3, bool occupancy_only, int *occ, hipStream_t s)
+                       bool share3, bool dense, bool occupancy_only, int *occ, hipStream_t s)
 {
 	switch (d->T.grid_type) {
 	case kUnionized:
-		return dispatch_bsorted_prec<kUnionized>(d, prec, B, q, stage, share3, occupancy_only, occ, s);
+		return dispatch_bsorted_prec<kUnionized>(d, prec, B, q, stage, share3, dense,
+		                                         occupancy_only, occ, s);
 	case kNuclide:
-		return dispatch_bsorted_prec<kNuclide>(d, prec, B, q, stage, share3, occupancy_only, occ, s);
+		return dispatch_bsorted_prec<kNuclide>(d, prec, B, q, stage, share3, false,
+		                                       occupancy_only, occ, s);
 	default:
-		return dispatch_bsorted_prec<kHash>(d, prec, B, q, stage, share3, occupancy_only, occ, s);
+		return dispatch_bsorted_prec<kHash>(d, prec, B, q, stage, share3, false, occupancy_only,
+		                                    occ, s);
 	}
 }
 
@@ -553,7 +568,8 @@ struct PipeShape {
 	int64_t n = 0;
 	int nb = 0, windows = 1, prec = 0, grid_type = 0;
 	BatchPlan plan;
-	int ebits = 0, passes = 1, grid = 0;
+	int ebits = 0, passes = 1, grid = 0, grid_dense = 0;
+	int dense_from = 1;  // windows [dense_from, windows) run the eight-wave variant
 	void *tables = nullptr, *sortbuf = nullptr;
 	bool operator==(const PipeShape &o) const
 	{
@@ -561,7 +577,8 @@ struct PipeShape {
 		       grid_type == o.grid_type && plan.tile == o.plan.tile &&
 		       plan.stage == o.plan.stage && plan.sort == o.plan.sort &&
 		       plan.mapping == o.plan.mapping && ebits == o.ebits &&
-		       passes == o.passes && grid == o.grid && tables == o.tables && sortbuf == o.sortbuf;
+		       passes == o.passes && grid == o.grid && grid_dense == o.grid_dense &&
+		       dense_from == o.dense_from && tables == o.tables && sortbuf == o.sortbuf;
 	}
 };
 
@@ -641,21 +658,57 @@ static int enqueue_pipeline(Device *d, const PipeShape &ps, const BatchView &B, 
 	q.tile = ps.plan.tile;
 	q.ebits = ps.ebits;
 	q.wave = ps.plan.mapping >= 2;
-	int grid = ps.grid;
-	if ((rc = run_bsorted(d, ps.prec, B, q, ps.plan.stage, ps.plan.mapping == 2, false, &grid, s)))
-		return rc;
+	// the sparse windows first (four waves per SIMD), then the dense ones
+	// (eight): each launch works through its own range of windows
+	if (ps.dense_from > 0) {
+		q.w_begin = 0;
+		q.w_end = ps.dense_from;
+		int grid = ps.grid;
+		if ((rc = run_bsorted(d, ps.prec, B, q, ps.plan.stage, ps.plan.mapping == 2, false, false,
+		                      &grid, s)))
+			return rc;
+	}
+	if (ps.dense_from < ps.windows) {
+		q.w_begin = ps.dense_from;
+		q.w_end = ps.windows;
+		int grid = ps.grid_dense;
+		if ((rc = run_bsorted(d, ps.prec, B, q, ps.plan.stage, ps.plan.mapping == 2, true, false,
+		                      &grid, s)))
+			return rc;
+	}
 	hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, clk + 2);
 	XSB_HIP(hipGetLastError());
 	return XSB_OK;
 }
 
-// Shape of one pipeline run over the first nb batches (total n lookups).
-static int make_shape(Device *d, int nb, int64_t n, int windows, int prec, const BatchPlan &plan,
-                      void *tables, PipeShape *ps)
+// Windows with at least 16 lookups per unionized energy are dense: their
+// waves mostly take the wave-uniform path, and eight waves per SIMD hide
+// its latency better (122M lookups in one window: 9.16 -> 8.17 ms); sparser
+// windows, whose straddling waves take the per-lane path, lose by it (that
+// path spills at 64 VGPRs: the small batches' window of the default plan
+// took 2.1 -> 3.4 ms).  The eight-wave variant runs only when every window
+// is dense: a launch of each variant for its own windows (sparse first) was
+// slower than one four-wave launch for all (12.97 against 12.04 ms).
+// Returns 0 (every window runs eight waves) or windows (none does).
+static int first_dense_window(const Device *d, const std::vector<uint32_t> &wpos, int windows)
+{
+	if (d->T.grid_type != kUnionized)
+		return windows;
+	for (int w = 0; w < windows; w++)
+		if ((int64_t)(wpos[w + 1] - wpos[w]) < 16 * (int64_t)d->T.n_union)
+			return windows;
+	return 0;
+}
+
+// Shape of one pipeline run over the first nb batches (total n lookups);
+// windows [dense_from, windows) run the eight-wave variant.
+static int make_shape(Device *d, int nb, int64_t n, int windows, int dense_from, int prec,
+                      const BatchPlan &plan, void *tables, PipeShape *ps)
 {
 	ps->n = n;
 	ps->nb = nb;
 	ps->windows = windows;
+	ps->dense_from = dense_from;
 	ps->prec = prec;
 	ps->grid_type = d->T.grid_type;
 	ps->plan = plan;
@@ -667,13 +720,16 @@ static int make_shape(Device *d, int nb, int64_t n, int windows, int prec, const
 	ps->ebits = plan.sort != 2 ? 0 : (wbits <= 2 ? 16 - wbits : std::min(16, 26 - wbits));
 	const int bits = wbits + 4 + ps->ebits;
 	ps->passes = (bits + kDigitBits - 1) / kDigitBits;
-	int occ = 0;
-	int rc = run_bsorted(d, prec, BatchView{}, SortedQueue{}, plan.stage, plan.mapping == 2, true,
-	                     &occ, nullptr);
-	if (rc)
-		return rc;
-	ps->grid = std::max(1, d->cu_count) * std::max(1, occ);
-	ps->grid = (ps->grid + kXcds - 1) / kXcds * kXcds;
+	for (int dense = 0; dense < 2; dense++) {
+		int occ = 0;
+		int rc = run_bsorted(d, prec, BatchView{}, SortedQueue{}, plan.stage, plan.mapping == 2,
+		                     dense != 0, true, &occ, nullptr);
+		if (rc)
+			return rc;
+		int &grid = dense ? ps->grid_dense : ps->grid;
+		grid = std::max(1, d->cu_count) * std::max(1, occ);
+		grid = (grid + kXcds - 1) / kXcds * kXcds;
+	}
 	ps->tables = tables;
 	const size_t sort_bytes =
 	    (size_t)(n + 1) * 16 + (size_t)sort_hist_len(n) * 4 + (size_t)sort_parts(n) * 4 + 256;
@@ -704,7 +760,8 @@ static int time_pipeline(Device *d, int nb, int64_t n, int prec, const BatchPlan
                          const TableLayout &L, float *ms)
 {
 	PipeShape ps;
-	int rc = make_shape(d, nb, n, 1, prec, plan, base, &ps);
+	const std::vector<uint32_t> w1{0u, (uint32_t)n};
+	int rc = make_shape(d, nb, n, 1, first_dense_window(d, w1, 1), prec, plan, base, &ps);
 	if (rc)
 		return rc;
 	BatchView B;
@@ -892,7 +949,8 @@ static int run_sorted_batches(Device *d, const xsb_batched_args *a, xsb_result *
 	r->kernel_id = 0;
 	if (total > 0) {
 		PipeShape ps;
-		if ((rc = make_shape(d, nb, total, windows, a->xs_precision, plan, base, &ps)))
+		if ((rc = make_shape(d, nb, total, windows, first_dense_window(d, wpos, windows),
+		                     a->xs_precision, plan, base, &ps)))
 			return rc;
 		BatchView B;
 		SortedQueue q;
